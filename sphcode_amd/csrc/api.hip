@@ -1,0 +1,685 @@
+// The extern "C" entry points of include/sphgpu.h.  Each call validates its
+// arguments, enqueues the stage's kernels on the context stream and reports
+// failures as negative status codes with a message (the reference throws
+// SPHException, include/exception.hpp:9-14).
+#include "ctx.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace sphgpu {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return set_error(SPHGPU_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+__global__ void set_scalar_kernel(double *scal, int slot, double v) { scal[slot] = v; }
+
+__global__ void fill_int_kernel(int *p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if(i < n) p[i] = v;
+}
+
+__global__ void iota_kernel(int *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if(i < n) p[i] = (int)i;
+}
+
+__global__ void fill_double_kernel(double *p, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if(i < n) p[i] = v;
+}
+
+__global__ void fill_i64_kernel(int64_t *p, int n)
+{
+    if((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+static bool field_used(const Ctx &c, int f)
+{
+    const int D = c.D;
+    if(f < F_MASS) return (f % 3) < D;
+    if(f >= F_GD && f < F_GD + 3) return c.p.sph_type == SPHGPU_GSPH && f - F_GD < D;
+    if(f >= F_GP && f < F_GP + 3) return c.p.sph_type == SPHGPU_GSPH && f - F_GP < D;
+    if(f >= F_GV && f < F_GV + 9) return c.p.sph_type == SPHGPU_GSPH && (f - F_GV) / 3 < D && (f - F_GV) % 3 < D;
+    return true;
+}
+
+static void free_buffers(Ctx &c)
+{
+    auto fr = [](auto *&p) {
+        if(p) (void)hipFree((void *)p);
+        p = nullptr;
+    };
+    for(int f = 0; f < NF; ++f) {
+        fr(c.s.f[f]);
+        fr(c.alt.f[f]);
+    }
+    fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
+    fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
+    fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
+    fr(c.parent); fr(c.nchild); fr(c.arrive); fr(c.msum);
+    fr(c.list); fr(c.list_r); fr(c.list_cnt);
+    fr(c.rec_gather); fr(c.rec_gather_cnt); fr(c.rec_force); fr(c.rec_force_cnt);
+    fr(c.partial);
+    c.cap_n = 0;
+}
+
+static int alloc_buffers(Ctx &c, int64_t n)
+{
+    free_buffers(c);
+    const int64_t np = (n + WAVE - 1) / WAVE * WAVE;
+    c.cap_n = np;
+    for(int f = 0; f < NF; ++f) {
+        if(!field_used(c, f)) continue;
+        SPHGPU_HIP(hipMalloc(&c.s.f[f], np * sizeof(double)));
+        SPHGPU_HIP(hipMalloc(&c.alt.f[f], np * sizeof(double)));
+    }
+    SPHGPU_HIP(hipMalloc(&c.s.id, np * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.alt.id, np * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.s.nb, np * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.alt.nb, np * sizeof(int)));
+    const size_t lsz = (size_t)np * (size_t)c.cap;
+    SPHGPU_HIP(hipMalloc(&c.list, lsz * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.list_r, lsz * sizeof(double)));
+    SPHGPU_HIP(hipMalloc(&c.list_cnt, np * sizeof(int)));
+    const int64_t npart = std::max<int64_t>(grid_for(np), 1024) * 8 + 64;
+    SPHGPU_HIP(hipMalloc(&c.partial, npart * sizeof(double)));
+    int rc = tree_alloc(c);
+    if(rc) return rc;
+    if(c.record) {
+        SPHGPU_HIP(hipMalloc(&c.rec_gather, lsz * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_gather_cnt, np * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_force, lsz * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_force_cnt, np * sizeof(int)));
+    }
+    return SPHGPU_OK;
+}
+
+// Reads the error counters; a list overflow or node-pool exhaustion since
+// the last check becomes an error status.
+static int check_flags(Ctx &c, const char *stage)
+{
+    SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    if(c.counters_host[C_TREE_ERR] > 0)
+        return set_error(SPHGPU_E_TREE, std::string(stage) + ": There is no free node.");
+    if(c.counters_host[C_OVERFLOW] > 0)
+        return set_error(SPHGPU_E_LIST_OVERFLOW, std::string(stage) + ": neighbour list capacity (" + std::to_string(c.cap) +
+                                                     ") exceeded; raise neighbor_capacity");
+    return SPHGPU_OK;
+}
+
+static int stage_tree(Ctx &c)
+{
+    int rc = tree_build(c);
+    if(rc) return rc;
+    c.tree_ready = true;
+    c.have_gather = c.have_force = false;
+    return SPHGPU_OK;
+}
+
+static int stage_pre(Ctx &c)
+{
+    int rc;
+    if(c.first_pre) {
+        // PreInteraction::initial_smoothing (src/pre_interaction.cpp:41-44, 171-215)
+        if((rc = search_gather(c, true))) return rc;
+        if((rc = pre_initial(c))) return rc;
+        c.first_pre = false;
+    }
+    if((rc = search_gather(c, false))) return rc;
+    if(c.record) {
+        const size_t lsz = (size_t)c.cap_n * (size_t)c.cap;
+        SPHGPU_HIP(hipMemcpyAsync(c.rec_gather, c.list, lsz * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+        SPHGPU_HIP(hipMemcpyAsync(c.rec_gather_cnt, c.list_cnt, c.cap_n * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+        c.have_gather = true;
+    }
+    if((rc = pre_compute(c))) return rc;
+    return tree_set_kernel(c);  // BHTree::set_kernel (src/pre_interaction.cpp:166-168)
+}
+
+static int stage_force(Ctx &c)
+{
+    int rc;
+    if((rc = search_symmetric(c))) return rc;
+    if((rc = force_compute(c))) return rc;
+    if(c.record) c.have_force = true;
+    return SPHGPU_OK;
+}
+
+}  // namespace sphgpu
+
+using namespace sphgpu;
+
+static Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+#define SPHGPU_CTX(ptr)                                                        \
+    if(!(ptr)) return set_error(SPHGPU_E_INVALID, "null context");           \
+    Ctx &c = *as_ctx(ptr);                                                     \
+    if(hipSetDevice(c.device) != hipSuccess) return set_error(SPHGPU_E_HIP, "hipSetDevice failed")
+
+#define SPHGPU_NEED_DATA()                                                     \
+    if(!c.uploaded) return set_error(SPHGPU_E_STATE, "no particles uploaded")
+
+#define SPHGPU_NEED_TREE()                                                     \
+    SPHGPU_NEED_DATA();                                                        \
+    if(!c.tree_ready) return set_error(SPHGPU_E_STATE, "tree not built")
+
+extern "C" {
+
+const char *sphgpu_last_error(void) { return g_error.c_str(); }
+
+int sphgpu_version(void) { return 1; }
+
+int sphgpu_params_default(sphgpu_params *p)
+{
+    if(!p) return set_error(SPHGPU_E_INVALID, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->dim = 3;
+    p->sph_type = SPHGPU_SSPH;
+    p->kernel = SPHGPU_CUBIC_SPLINE;
+    p->precision = SPHGPU_FP64;
+    p->cfl_sound = 0.3;
+    p->cfl_force = 0.125;
+    p->av_alpha = 1.0;
+    p->av_use_balsara_switch = 1;
+    p->av_use_time_dependent = 0;
+    p->av_alpha_max = 2.0;
+    p->av_alpha_min = 0.1;
+    p->av_epsilon = 0.2;
+    p->ac_alpha = 1.0;
+    p->ac_is_valid = 0;
+    p->tree_max_level = 20;
+    p->tree_leaf_particle_num = 1;
+    p->neighbor_number = 32;
+    p->iterative_sml = 1;
+    p->gravity_constant = 1.0;
+    p->gravity_theta = 0.5;
+    p->gsph_is_2nd_order = 1;
+    return SPHGPU_OK;
+}
+
+int sphgpu_create(const sphgpu_params *params, void **out)
+{
+    if(!params || !out) return set_error(SPHGPU_E_INVALID, "null argument");
+    const sphgpu_params &p = *params;
+    if(p.dim < 1 || p.dim > 3) return set_error(SPHGPU_E_INVALID, "dim must be 1, 2 or 3");
+    if(p.sph_type < 0 || p.sph_type > 2) return set_error(SPHGPU_E_INVALID, "Unknown SPH type");
+    if(p.kernel != SPHGPU_CUBIC_SPLINE && p.kernel != SPHGPU_WENDLAND) return set_error(SPHGPU_E_INVALID, "kernel is unknown.");
+    if(p.precision != SPHGPU_FP64 && p.precision != SPHGPU_MIXED) return set_error(SPHGPU_E_INVALID, "unknown precision");
+    if(p.tree_max_level < 1 || p.tree_max_level * p.dim > 63)
+        return set_error(SPHGPU_E_INVALID, "maxTreeLevel must satisfy 1 <= maxTreeLevel and maxTreeLevel * DIM <= 63");
+    if(p.tree_leaf_particle_num < 1) return set_error(SPHGPU_E_INVALID, "leafParticleNumber must be >= 1");
+    if(p.neighbor_number < 1) return set_error(SPHGPU_E_INVALID, "neighborNumber must be >= 1");
+    if(p.av_use_time_dependent && p.av_alpha_max < p.av_alpha_min) return set_error(SPHGPU_E_INVALID, "alphaMax < alphaMin");
+    if(p.time_end < p.time_start) return set_error(SPHGPU_E_INVALID, "endTime < startTime");
+    if(p.periodic_is_valid)
+        for(int k = 0; k < p.dim; ++k)
+            if(!(p.range_max[k] > p.range_min[k])) return set_error(SPHGPU_E_INVALID, "periodic range is empty");
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(SPHGPU_E_NO_DEVICE, "no HIP device");
+    Ctx *c = new Ctx();
+    c->p = p;
+    c->D = p.dim;
+    c->cap = p.neighbor_capacity > 0 ? p.neighbor_capacity : p.neighbor_number * 20;
+    c->time = p.time_start;
+    c->dt = 0.0;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    c->device = dev;
+    hipError_t e;
+    if((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+       (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
+       (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
+       (e = hipHostMalloc(&c->counters_host, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
+       (e = hipHostMalloc(&c->scal_host, NSCALARS * sizeof(double), hipHostMallocDefault)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "sphgpu_create");
+    }
+    for(int k = 0; k < 16; ++k) (void)hipEventCreate(&c->ev[k]);
+    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c->stream, c->counters, NCOUNTERS);
+    hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->scal, (int64_t)NSCALARS, 0.0);
+    if(p.periodic_is_valid) {
+        // BHTree::initialize (src/bhtree.cpp:19-31): root = the periodic box
+        double edge = 0.0;
+        for(int k = 0; k < p.dim; ++k) {
+            hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c->stream, c->scal, S_ROOT_CX + k,
+                               (p.range_max[k] + p.range_min[k]) * 0.5);
+            const double r = p.range_max[k] - p.range_min[k];
+            if(edge < r) edge = r;
+        }
+        hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c->stream, c->scal, (int)S_ROOT_EDGE, edge);
+    }
+    if((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "sphgpu_create");
+    }
+    *out = c;
+    return SPHGPU_OK;
+}
+
+int sphgpu_destroy(void *ptr)
+{
+    if(!ptr) return SPHGPU_OK;
+    Ctx *c = as_ctx(ptr);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_buffers(*c);
+    if(c->scal) (void)hipFree(c->scal);
+    if(c->counters) (void)hipFree(c->counters);
+    if(c->counters_host) (void)hipHostFree(c->counters_host);
+    if(c->scal_host) (void)hipHostFree(c->scal_host);
+    for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
+    if(c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_record_sets(void *ptr, int enable)
+{
+    SPHGPU_CTX(ptr);
+    c.record = enable != 0;
+    if(c.uploaded && c.record && !c.rec_gather) {
+        const size_t lsz = (size_t)c.cap_n * (size_t)c.cap;
+        SPHGPU_HIP(hipMalloc(&c.rec_gather, lsz * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_gather_cnt, c.cap_n * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_force, lsz * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.rec_force_cnt, c.cap_n * sizeof(int)));
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_upload(void *ptr, const sphgpu_particles *h)
+{
+    SPHGPU_CTX(ptr);
+    if(!h || h->n <= 0) return set_error(SPHGPU_E_INVALID, "empty particle array");
+    if(h->n > (int64_t)0x7fffffff / 2) return set_error(SPHGPU_E_INVALID, "too many particles");
+    if(!h->pos || !h->mass) return set_error(SPHGPU_E_INVALID, "pos and mass are required");
+    const int64_t n = h->n;
+    const int D = c.D;
+    if(n != c.n || !c.s.f[F_X]) {
+        int rc = alloc_buffers(c, n);
+        if(rc) return rc;
+    }
+    c.n = n;
+    std::vector<double> tmp(n);
+    auto put = [&](int f, const double *src, int stride, int comp) -> int {
+        if(!c.s.f[f]) return SPHGPU_OK;
+        if(!src) {
+            hipLaunchKernelGGL(fill_double_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s.f[f], n, 0.0);
+            return SPHGPU_OK;
+        }
+        for(int64_t i = 0; i < n; ++i) tmp[i] = src[i * stride + comp];
+        SPHGPU_HIP(hipMemcpy(c.s.f[f], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        return SPHGPU_OK;
+    };
+    int rc = 0;
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    for(int k = 0; k < D && !rc; ++k) {
+        rc = rc ? rc : put(F_X + k, h->pos, D, k);
+        rc = rc ? rc : put(F_V + k, h->vel, D, k);
+        rc = rc ? rc : put(F_VP + k, h->vel_p, D, k);
+        rc = rc ? rc : put(F_A + k, h->acc, D, k);
+        rc = rc ? rc : put(F_GD + k, h->grad_density, D, k);
+        rc = rc ? rc : put(F_GP + k, h->grad_pressure, D, k);
+        for(int l = 0; l < D && !rc; ++l) rc = put(F_GV + 3 * k + l, h->grad_velocity, D * D, k * D + l);
+    }
+    const double *sc[] = {h->mass, h->dens, h->pres, h->ene, h->ene_p, h->dene, h->sml, h->sound, h->balsara, h->alpha, h->gradh, h->phi};
+    for(int f = F_MASS; f <= F_PHI && !rc; ++f) rc = put(f, sc[f - F_MASS], 1, 0);
+    if(!rc && c.s.f[F_HINV]) rc = put(F_HINV, nullptr, 1, 0);
+    if(rc) return rc;
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s.id, n);
+    if(h->neighbor) {
+        SPHGPU_HIP(hipMemcpy(c.s.nb, h->neighbor, n * sizeof(int), hipMemcpyHostToDevice));
+    } else {
+        hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s.nb, n, 0);
+    }
+    // list slots beyond n stay empty
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c.cap_n)), dim3(BLOCK), 0, c.stream, c.list_cnt, c.cap_n, 0);
+    SPHGPU_HIP(hipGetLastError());
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    c.uploaded = true;
+    c.first_pre = true;
+    c.tree_ready = false;
+    c.have_gather = c.have_force = false;
+    return SPHGPU_OK;
+}
+
+int sphgpu_download(void *ptr, sphgpu_particles *h)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    if(!h || h->n != c.n) return set_error(SPHGPU_E_INVALID, "particle count mismatch");
+    const int64_t n = c.n;
+    const int D = c.D;
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> id(n);
+    SPHGPU_HIP(hipMemcpy(id.data(), c.s.id, n * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> tmp(n);
+    auto get = [&](int f, double *dst, int stride, int comp) -> int {
+        if(!dst) return SPHGPU_OK;
+        if(!c.s.f[f]) {
+            for(int64_t i = 0; i < n; ++i) dst[(int64_t)id[i] * stride + comp] = 0.0;
+            return SPHGPU_OK;
+        }
+        SPHGPU_HIP(hipMemcpy(tmp.data(), c.s.f[f], n * sizeof(double), hipMemcpyDeviceToHost));
+        for(int64_t i = 0; i < n; ++i) dst[(int64_t)id[i] * stride + comp] = tmp[i];
+        return SPHGPU_OK;
+    };
+    int rc = 0;
+    for(int k = 0; k < D && !rc; ++k) {
+        rc = rc ? rc : get(F_X + k, h->pos, D, k);
+        rc = rc ? rc : get(F_V + k, h->vel, D, k);
+        rc = rc ? rc : get(F_VP + k, h->vel_p, D, k);
+        rc = rc ? rc : get(F_A + k, h->acc, D, k);
+        rc = rc ? rc : get(F_GD + k, h->grad_density, D, k);
+        rc = rc ? rc : get(F_GP + k, h->grad_pressure, D, k);
+        for(int l = 0; l < D && !rc; ++l) rc = get(F_GV + 3 * k + l, h->grad_velocity, D * D, k * D + l);
+    }
+    double *sc[] = {h->mass, h->dens, h->pres, h->ene, h->ene_p, h->dene, h->sml, h->sound, h->balsara, h->alpha, h->gradh, h->phi};
+    for(int f = F_MASS; f <= F_PHI && !rc; ++f) rc = get(f, sc[f - F_MASS], 1, 0);
+    if(rc) return rc;
+    if(h->id)
+        for(int64_t i = 0; i < n; ++i) h->id[id[i]] = id[i];
+    if(h->neighbor) {
+        std::vector<int> nb(n);
+        SPHGPU_HIP(hipMemcpy(nb.data(), c.s.nb, n * sizeof(int), hipMemcpyDeviceToHost));
+        for(int64_t i = 0; i < n; ++i) h->neighbor[id[i]] = nb[i];
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_make_tree(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    int rc = stage_tree(c);
+    return rc ? rc : check_flags(c, "make_tree");
+}
+
+int sphgpu_pre_interaction(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_TREE();
+    int rc = stage_pre(c);
+    if(rc) return rc;
+    if((rc = check_flags(c, "pre_interaction"))) return rc;
+    SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    c.h_per_v_sig = c.scal_host[S_HVS];
+    return SPHGPU_OK;
+}
+
+int sphgpu_fluid_force(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_TREE();
+    int rc = stage_force(c);
+    return rc ? rc : check_flags(c, "fluid_force");
+}
+
+int sphgpu_gravity_force(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_TREE();
+    int rc = gravity_compute(c);
+    return rc ? rc : check_flags(c, "gravity_force");
+}
+
+int sphgpu_timestep(void *ptr, double *dt_out)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    int rc = step_timestep(c);
+    if(rc) return rc;
+    SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    c.dt = c.scal_host[S_DT];
+    if(dt_out) *dt_out = c.dt;
+    return SPHGPU_OK;
+}
+
+int sphgpu_predict(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    return step_predict(c);
+}
+
+int sphgpu_correct(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    return step_correct(c);
+}
+
+int sphgpu_initialize(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    int rc;
+    if((rc = step_init_fields(c))) return rc;
+    if((rc = sphgpu_make_tree(ptr))) return rc;
+    if((rc = sphgpu_pre_interaction(ptr))) return rc;
+    if((rc = sphgpu_fluid_force(ptr))) return rc;
+    return sphgpu_gravity_force(ptr);
+}
+
+int sphgpu_integrate(void *ptr, double fixed_dt)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_TREE();
+    int rc;
+    hipEvent_t *ev = c.ev;
+    auto mark = [&](int k) {
+        if(c.timers_on) (void)hipEventRecord(ev[k], c.stream);
+    };
+    mark(0);
+    if(fixed_dt > 0.0) hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, fixed_dt);
+    else if((rc = step_timestep(c))) return rc;
+    mark(1);
+    if((rc = step_predict(c))) return rc;
+    mark(2);
+    if((rc = stage_tree(c))) return rc;
+    mark(3);
+    if((rc = stage_pre(c))) return rc;
+    mark(4);
+    if((rc = stage_force(c))) return rc;
+    mark(5);
+    if((rc = gravity_compute(c))) return rc;
+    mark(6);
+    if((rc = step_correct(c))) return rc;
+    mark(7);
+    SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if((rc = check_flags(c, "integrate"))) return rc;  // synchronises the stream
+    c.dt = c.scal_host[S_DT];
+    c.h_per_v_sig = c.scal_host[S_HVS];
+    c.time += c.dt;
+    ++c.steps;
+    if(c.timers_on) {
+        for(int k = 0; k < 7; ++k) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            c.timer_ms[k] += ms;
+        }
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_time(void *ptr, double *time, double *dt, double *hvs)
+{
+    SPHGPU_CTX(ptr);
+    if(time) *time = c.time;
+    if(dt) *dt = c.dt;
+    if(hvs) *hvs = c.h_per_v_sig;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_time(void *ptr, double time, double dt)
+{
+    SPHGPU_CTX(ptr);
+    c.time = time;
+    c.dt = dt;
+    hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, dt);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int sphgpu_energy(void *ptr, double *out)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    if(!out) return set_error(SPHGPU_E_INVALID, "null output");
+    return step_energy(c, out);
+}
+
+int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
+{
+    SPHGPU_CTX(ptr);
+    if(!st) return set_error(SPHGPU_E_INVALID, "null output");
+    SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    const int64_t *k = c.counters_host;
+    st->steps = c.steps;
+    st->gather_pairs = k[C_GATHER_PAIRS];
+    st->force_pairs = k[C_FORCE_PAIRS];
+    st->gravity_pp = k[C_GRAV_PP];
+    st->gravity_pn = k[C_GRAV_PN];
+    st->newton_failures = k[C_NEWTON_FAIL];
+    st->newton_iterations = k[C_NEWTON_ITERS];
+    st->search_truncations = k[C_TRUNC];
+    st->list_overflows = k[C_OVERFLOW];
+    st->max_list_length = k[C_MAXLIST];
+    int nodes = 0;
+    if(c.tree_ready) SPHGPU_HIP(hipMemcpy(&nodes, c.t.num_nodes, sizeof(int), hipMemcpyDeviceToHost));
+    st->tree_nodes = nodes;
+    st->tree_max_level = 0;
+    if(c.tree_ready && nodes > 0) {
+        std::vector<int4> rng(nodes);
+        SPHGPU_HIP(hipMemcpy(rng.data(), c.t.rng, nodes * sizeof(int4), hipMemcpyDeviceToHost));
+        int ml = 1;
+        for(const int4 &r : rng) ml = std::max(ml, r.w & 255);
+        st->tree_max_level = ml;
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_timers(void *ptr, sphgpu_timers *t)
+{
+    SPHGPU_CTX(ptr);
+    if(!t) return set_error(SPHGPU_E_INVALID, "null output");
+    t->timestep = c.timer_ms[0];
+    t->predict = c.timer_ms[1];
+    t->tree = c.timer_ms[2];
+    t->pre_interaction = c.timer_ms[3];
+    t->fluid_force = c.timer_ms[4];
+    t->gravity = c.timer_ms[5];
+    t->correct = c.timer_ms[6];
+    return SPHGPU_OK;
+}
+
+int sphgpu_reset_stats(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.counters, NCOUNTERS);
+    SPHGPU_HIP(hipGetLastError());
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    for(double &x : c.timer_ms) x = 0.0;
+    c.steps = 0;
+    return SPHGPU_OK;
+}
+
+int sphgpu_enable_timers(void *ptr, int enable)
+{
+    SPHGPU_CTX(ptr);
+    c.timers_on = enable != 0;
+    return SPHGPU_OK;
+}
+
+int sphgpu_neighbor_sets(void *ptr, int kind, int32_t *counts, int32_t *lists, int64_t cap)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    if(!counts || !lists) return set_error(SPHGPU_E_INVALID, "null output");
+    if((kind == 0 && !c.have_gather) || (kind != 0 && !c.have_force))
+        return set_error(SPHGPU_E_STATE, "sets not recorded (enable sphgpu_set_record_sets before the stage)");
+    const int64_t n = c.n;
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> id(n), cnt(n);
+    const size_t lsz = (size_t)c.cap_n * (size_t)c.cap;
+    std::vector<int> lst(lsz);
+    SPHGPU_HIP(hipMemcpy(id.data(), c.s.id, n * sizeof(int), hipMemcpyDeviceToHost));
+    SPHGPU_HIP(hipMemcpy(cnt.data(), kind == 0 ? c.rec_gather_cnt : c.rec_force_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+    SPHGPU_HIP(hipMemcpy(lst.data(), kind == 0 ? c.rec_gather : c.rec_force, lsz * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<std::vector<int>> sets(n);
+    for(int64_t i = 0; i < n; ++i) {
+        const size_t base = (size_t)(i / WAVE) * c.cap * WAVE + i % WAVE;
+        std::vector<int> &m = sets[id[i]];
+        m.resize(cnt[i]);
+        for(int k = 0; k < cnt[i]; ++k) m[k] = id[lst[base + (size_t)k * WAVE]];
+        std::sort(m.begin(), m.end());
+    }
+    int64_t off = 0;
+    for(int64_t i = 0; i < n; ++i) {
+        counts[i] = (int32_t)sets[i].size();
+        for(int j : sets[i]) {
+            if(off >= cap) return set_error(SPHGPU_E_INVALID, "list buffer too small");
+            lists[off++] = j;
+        }
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_tree_info(void *ptr, int64_t *num_nodes, int32_t *level, int32_t *count, double *edge, double *center, double *mass,
+                     double *com, int64_t cap)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_TREE();
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    int nodes = 0;
+    SPHGPU_HIP(hipMemcpy(&nodes, c.t.num_nodes, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int4> rng(nodes);
+    std::vector<double4> geo(nodes), mom(nodes);
+    SPHGPU_HIP(hipMemcpy(rng.data(), c.t.rng, nodes * sizeof(int4), hipMemcpyDeviceToHost));
+    SPHGPU_HIP(hipMemcpy(geo.data(), c.t.geo, nodes * sizeof(double4), hipMemcpyDeviceToHost));
+    SPHGPU_HIP(hipMemcpy(mom.data(), c.t.mom, nodes * sizeof(double4), hipMemcpyDeviceToHost));
+    const int D = c.D;
+    for(int64_t k = 0; k < nodes && k < cap; ++k) {
+        if(level) level[k] = rng[k].w & 255;
+        if(count) count[k] = rng[k].y - rng[k].x;
+        if(edge) edge[k] = geo[k].w;
+        const double cc[3] = {geo[k].x, geo[k].y, geo[k].z};
+        const double cm[3] = {mom[k].x, mom[k].y, mom[k].z};
+        for(int d = 0; d < D; ++d) {
+            if(center) center[k * D + d] = cc[d];
+            if(com) com[k * D + d] = cm[d];
+        }
+        if(mass) mass[k] = mom[k].w;
+    }
+    *num_nodes = nodes;
+    return SPHGPU_OK;
+}
+
+int sphgpu_synchronize(void *ptr)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    return SPHGPU_OK;
+}
+
+}  // extern "C"

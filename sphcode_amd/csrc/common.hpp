@@ -1,0 +1,251 @@
+// Device-side primitives shared by every stage: wave64 helpers, periodic
+// minimum image, SPH kernels and the Hernquist-Katz gravitational softening.
+//
+// Distances that decide set membership (neighbour candidates, density
+// cut-off, force cut-off, cell pruning) are evaluated in the reference's
+// operation order with FP contraction disabled, so FP64 sets match the
+// reference's exactly.  Sums over pairs may be contracted.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphgpu {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+
+// ------------------------------------------------------------- wave64 helpers
+__device__ __forceinline__ int wave_lane() { return __lane_id(); }
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+__device__ __forceinline__ int wave_min_int(int v)
+{
+    for(int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <class T> __device__ __forceinline__ T wave_sum(T v)
+{
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_minv(T v)
+{
+    for(int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off, WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_maxv(T v)
+{
+    for(int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// --------------------------------------------------------------- scalar math
+template <class T> __host__ __device__ __forceinline__ T sq(T x) { return x * x; }
+template <class T> __host__ __device__ __forceinline__ T cube(T x) { return x * x * x; }
+
+// h^D (include/kernel/kernel_function.hpp:8-16)
+template <int D, class T> __device__ __forceinline__ T pow_dim(T h)
+{
+    if constexpr(D == 1) return h;
+    else if constexpr(D == 2) return h * h;
+    else return h * h * h;
+}
+
+// h^(D-1) (src/pre_interaction.cpp:217-225)
+template <int D, class T> __device__ __forceinline__ T pow_dim1(T h)
+{
+    if constexpr(D == 1) return T(1);
+    else if constexpr(D == 2) return h;
+    else return h * h;
+}
+
+// x^(1/D); the reference's -ffast-math build evaluates std::pow(x, 1.0/DIM)
+// (src/pre_interaction.cpp:64) as x, sqrt(x), cbrt(x).
+template <int D> __device__ __forceinline__ double root_dim(double x)
+{
+    if constexpr(D == 1) return x;
+    else if constexpr(D == 2) return sqrt(x);
+    else return cbrt(x);
+}
+
+// Unit-sphere volume A_D (src/pre_interaction.cpp:61-63).
+template <int D> struct UnitVolume {
+    static constexpr double value = D == 1 ? 2.0 : D == 2 ? 3.14159265358979323846 : 4.0 * 3.14159265358979323846 / 3.0;
+};
+
+// -------------------------------------------------------- periodic boundary
+// Periodic::calc_r_ij (include/periodic.hpp:34-59): per axis, the smallest
+// of |d|, |d+L|, |d-L| in the reference's tie order.
+struct BoxDev {
+    int on;
+    double lo[3], hi[3], len[3];
+};
+
+template <int D> __device__ __forceinline__ void min_image(const BoxDev &b, double d[D])
+{
+#pragma clang fp contract(off)
+    if(!b.on) return;
+#pragma unroll
+    for(int k = 0; k < D; ++k) {
+        const double d1 = d[k];
+        const double d2 = d1 + b.len[k];
+        const double d3 = d1 - b.len[k];
+        const double a1 = fabs(d1), a2 = fabs(d2), a3 = fabs(d3);
+        double r = d3;
+        if(a2 <= a3 && a2 <= a1) r = d2;
+        if(a1 <= a2 && a1 <= a3) r = d1;
+        d[k] = r;
+    }
+}
+
+// r_ij = x_i - x_j with minimum image, and its squared norm summed left to
+// right without contraction (include/vector_type.hpp:194-227).
+template <int D> __device__ __forceinline__ double sep2(const BoxDev &b, const double xi[D], const double xj[D], double d[D])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for(int k = 0; k < D; ++k) d[k] = xi[k] - xj[k];
+    min_image<D>(b, d);
+    double r2 = d[0] * d[0];
+#pragma unroll
+    for(int k = 1; k < D; ++k) r2 = r2 + d[k] * d[k];
+    return r2;
+}
+
+// Periodic::apply (include/periodic.hpp:61-72): one-period wrap.
+template <int D> __device__ __forceinline__ void wrap(const BoxDev &b, double x[D])
+{
+#pragma clang fp contract(off)
+    if(!b.on) return;
+#pragma unroll
+    for(int k = 0; k < D; ++k) {
+        if(x[k] < b.lo[k]) x[k] += b.len[k];
+        else if(x[k] > b.hi[k]) x[k] -= b.len[k];
+    }
+}
+
+// ------------------------------------------------------------- SPH kernels
+// Both kernels are evaluated per pair from q and per-particle constants of
+// h: for W, grad W = r_ij * coef and dW/dh.  Formulas follow
+// include/kernel/cubic_spline.hpp:21-52 and
+// include/kernel/wendland_kernel.hpp:23-49; the per-h constants are
+// hoisted out of the pair loop.
+
+// Cubic spline with support h: H = h/2, q = r/H < 2.
+template <int D, class T> struct Cubic {
+    static constexpr double sigma_d = D == 1 ? 2.0 / 3.0 : D == 2 ? 10.0 / (7.0 * 3.14159265358979323846) : 1.0 / 3.14159265358979323846;
+    T qscale;   // 1/H
+    T wnorm;    // sigma / H^D
+    T gnorm;    // sigma / (H^D H)
+    __device__ __forceinline__ explicit Cubic(T h) : Cubic(h, T(1) / h) {}
+    // hinv = 1/h supplied by the caller
+    __device__ __forceinline__ Cubic(T /*h*/, T hinv)
+    {
+        qscale = T(2) * hinv;
+        wnorm = T(sigma_d) * pow_dim<D>(qscale);
+        gnorm = wnorm * qscale;
+    }
+    __device__ __forceinline__ T q_of(T r) const { return r * qscale; }
+    __device__ __forceinline__ T w(T q) const
+    {
+        const T a = fmax(T(2) - q, T(0)), b = fmax(T(1) - q, T(0));
+        return wnorm * (T(0.25) * a * a * a - b * b * b);
+    }
+    // grad W = r_ij * dw_coef(q, r)
+    __device__ __forceinline__ T dw_coef(T q, T r) const
+    {
+        if(r == T(0)) return T(0);
+        const T a = fmax(T(2) - q, T(0)), b = fmax(T(1) - q, T(0));
+        return -gnorm / r * (T(0.75) * a * a - T(3) * b * b);
+    }
+    __device__ __forceinline__ T dhw(T q) const
+    {
+        const T a = fmax(T(2) - q, T(0)), b = fmax(T(1) - q, T(0));
+        return T(0.5) * gnorm * (a * a * (T((3.0 + D) * 0.25) * q - T(0.5 * D)) + b * b * (T(-3.0 - D) * q + T(D)));
+    }
+};
+
+// Wendland C4 with support h, q = r/h < 1 (Dehnen & Aly 2012).
+template <int D, class T> struct WendlandC4 {
+    static constexpr double sigma_d = D == 1 ? 0.0 : D == 2 ? 9.0 / 3.14159265358979323846 : 495.0 / (32.0 * 3.14159265358979323846);
+    T qscale;   // 1/h
+    T wnorm;    // sigma / h^D
+    T gnorm;    // 56/3 sigma / (h^D h^2)
+    T hnorm;    // sigma / (3 h^D h)
+    __device__ __forceinline__ explicit WendlandC4(T h) : WendlandC4(h, T(1) / h) {}
+    // hinv = 1/h supplied by the caller
+    __device__ __forceinline__ WendlandC4(T /*h*/, T hinv)
+    {
+        qscale = hinv;
+        wnorm = T(sigma_d) * pow_dim<D>(hinv);
+        gnorm = T(56.0 / 3.0) * wnorm * qscale * qscale;
+        hnorm = wnorm * qscale * T(1.0 / 3.0);
+    }
+    __device__ __forceinline__ T q_of(T r) const { return r * qscale; }
+    __device__ __forceinline__ T w(T q) const
+    {
+        const T t = fmax(T(1) - q, T(0));
+        const T t2 = t * t;
+        return wnorm * (t2 * t2 * t2) * (T(1) + T(6) * q + T(35.0 / 3.0) * q * q);
+    }
+    __device__ __forceinline__ T dw_coef(T q, T /*r*/) const
+    {
+        const T t = fmax(T(1) - q, T(0));
+        const T t2 = t * t;
+        return -gnorm * (t2 * t2 * t) * (T(1) + T(5) * q);
+    }
+    __device__ __forceinline__ T dhw(T q) const
+    {
+        const T t = fmax(T(1) - q, T(0));
+        const T t2 = t * t;
+        return -hnorm * (t2 * t2 * t) *
+               (T(3 * D) + T(15 * D) * q + T(-56 + 17 * D) * q * q - T(35 * (8 + D)) * q * q * q);
+    }
+};
+
+template <int D, int K, class T> struct KernelOf;
+template <int D, class T> struct KernelOf<D, 0, T> { using type = Cubic<D, T>; };
+template <int D, class T> struct KernelOf<D, 1, T> { using type = WendlandC4<D, T>; };
+
+// ------------------------------------------- Hernquist & Katz softening
+// f = potential kernel and g = force kernel of src/bhtree.cpp:273-299 with
+// e = h/2, u = r/e; g(r) * r_ij is the softened r_ij / r^3.
+template <class T> __device__ __forceinline__ T soft_pot(T r, T einv)
+{
+    const T u = r * einv;
+    const T u2 = u * u;
+    if(u < T(1)) return (T(-0.5) * u2 * (T(1.0 / 3.0) - T(3.0 / 20.0) * u2 + u2 * u * T(1.0 / 20.0)) + T(1.4)) * einv;
+    if(u < T(2)) return T(-1.0 / 15.0) / r + (-u2 * (T(4.0 / 3.0) - u + T(0.3) * u2 - u2 * u * T(1.0 / 30.0)) + T(1.6)) * einv;
+    return T(1) / r;
+}
+
+template <class T> __device__ __forceinline__ T soft_acc(T r, T einv)
+{
+    const T u = r * einv;
+    const T u2 = u * u;
+    if(u < T(1)) return (T(4.0 / 3.0) - T(1.2) * u2 + T(0.5) * u2 * u) * (einv * einv * einv);
+    const T r3inv = T(1) / (r * r * r);
+    if(u < T(2)) {
+        const T u3 = u2 * u;
+        return (T(-1.0 / 15.0) + T(8.0 / 3.0) * u3 - T(3) * u3 * u + T(1.2) * u3 * u2 - u3 * u3 * T(1.0 / 6.0)) * r3inv;
+    }
+    return r3inv;
+}
+
+}  // namespace sphgpu

@@ -1,0 +1,146 @@
+// Device context: particle state (structure of arrays, FP64 master copy in
+// Morton order), the octree, neighbour lists and counters, plus the host
+// launchers of every stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+
+#include "../../include/sphgpu.h"
+#include "common.hpp"
+
+namespace sphgpu {
+
+// Per-particle FP64 fields.  Vectors occupy three consecutive slots
+// (axis k at base + k); only the first D are allocated.
+enum Field : int {
+    F_X = 0, F_V = 3, F_VP = 6, F_A = 9,
+    F_MASS = 12, F_DENS, F_PRES, F_ENE, F_ENEP, F_DENE, F_SML, F_SOUND,
+    F_BAL, F_ALPHA, F_GRADH, F_PHI,
+    F_GD = 24, F_GP = 27, F_GV = 30,   // GSPH gradients; grad v_k along l at F_GV + 3k + l
+    F_HINV = 39,                        // 1/h, written with h by the pre-interaction
+    NF = 40
+};
+
+// Kernel-argument view of the state.
+struct State {
+    double *f[NF];
+    int *id;   // caller index of the particle in each sorted slot
+    int *nb;   // neighbour count (SPHParticle::neighbor)
+};
+
+// Octree in depth-first pre-order; node 0 is the root.
+struct TreeDev {
+    double4 *geo;    // centre (x,y,z), edge
+    double4 *mom;    // centre of mass (x,y,z), mass
+    double *ks;      // largest h in the subtree (BHNode::kernel_size)
+    int4 *rng;       // first particle, end particle, skip (pre-order successor
+                     // after the subtree), level | leaf << 8
+    int *num_nodes;  // device scalar
+};
+
+// Device-side counters (int64), reduced per wave before the atomic.
+enum Counter : int {
+    C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, NCOUNTERS
+};
+
+// Device scalars.
+enum Scalar : int {
+    S_DT = 0, S_HVS, S_ROOT_CX, S_ROOT_CY, S_ROOT_CZ, S_ROOT_EDGE, NSCALARS
+};
+
+struct Ctx {
+    sphgpu_params p{};
+    int D = 3;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    int64_t n = 0;        // particles
+    int64_t cap_n = 0;    // allocated particles
+    int cap = 0;          // per-particle list capacity
+    int64_t node_cap = 0; // allocated nodes
+
+    State s{};       // current buffers
+    State alt{};     // reorder targets
+
+    // tree build
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    int *idx = nullptr, *perm = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    uint8_t *delta = nullptr;
+    int *nn = nullptr, *noff = nullptr;
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    TreeDev t{};
+    int *parent = nullptr, *nchild = nullptr, *arrive = nullptr;
+    double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
+
+    // neighbour lists: list[(wave * cap + k) * 64 + lane]
+    int *list = nullptr;
+    double *list_r = nullptr;  // |r_ij| of gather candidates
+    int *list_cnt = nullptr;
+    int *rec_gather = nullptr, *rec_gather_cnt = nullptr;  // recorded sets (debug)
+    int *rec_force = nullptr, *rec_force_cnt = nullptr;
+    bool record = false;
+    bool have_gather = false, have_force = false;
+
+    double *scal = nullptr;     // device scalars
+    double *partial = nullptr;  // block partials of reductions
+    int64_t *counters = nullptr;
+    int64_t *counters_host = nullptr;  // pinned
+    double *scal_host = nullptr;       // pinned
+
+    // host mirror of simulation scalars
+    double time = 0.0, dt = 0.0, h_per_v_sig = 0.0;
+    bool uploaded = false, first_pre = true, tree_ready = false;
+
+    // stage timers
+    bool timers_on = false;
+    hipEvent_t ev[16];
+    double timer_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t steps = 0;
+    int64_t last_nodes = 0, last_levels = 0;
+};
+
+// Error state (per thread).
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define SPHGPU_HIP(call)                                              \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if(e_ != hipSuccess) return ::sphgpu::hip_fail(e_, #call);    \
+    } while(0)
+
+inline int grid_for(int64_t n, int block = BLOCK) { return (int)((n + block - 1) / block); }
+
+BoxDev make_box(const Ctx &c);
+
+// ------------------------------------------------------------- launchers
+// tree.hip
+int tree_alloc(Ctx &c);
+int tree_build(Ctx &c);          // BHTree::make
+int tree_set_kernel(Ctx &c);     // BHTree::set_kernel
+// search.hip: candidate lists from the tree walk
+int search_gather(Ctx &c, bool initial);  // h from the density guess
+int search_symmetric(Ctx &c);
+// pre.hip
+int pre_initial(Ctx &c);         // initial_smoothing density
+int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV switches / MUSCL gradients
+// force.hip
+int force_compute(Ctx &c);
+// gravity.hip
+int gravity_compute(Ctx &c);
+// step.hip
+int step_init_fields(Ctx &c);
+int step_timestep(Ctx &c);       // dt into scal[S_DT]
+int step_predict(Ctx &c);
+int step_correct(Ctx &c);
+int step_energy(Ctx &c, double *out);
+int reduce_min_partials(Ctx &c, int nblocks, int slot);
+
+}  // namespace sphgpu

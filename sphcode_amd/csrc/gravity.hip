@@ -1,0 +1,135 @@
+// GravityForce on the device: Barnes-Hut walk per particle with the
+// reference's opening test and softening (GravityForce::calculation,
+// src/gravity_force.cpp:52-89 -> BHNode::calc_force, src/bhtree.cpp:301-331).
+//
+// A cell is opened when edge^2 > theta^2 |x_i - com|^2; an opened leaf is
+// summed particle by particle, i itself included (the self-potential), with
+// the Hernquist & Katz spline softening averaged over h_i and h_j; an
+// accepted cell adds its monopole.  phi is reset and acc is accumulated on
+// top of the hydrodynamic acceleration.  The walk is the wave-coherent
+// pre-order walk of walk.hpp, so each lane keeps its own opening decisions.
+#include "walk.hpp"
+
+namespace sphgpu {
+
+struct GravArgs {
+    double G;
+    double theta2;
+    int64_t *counters;
+};
+
+template <int D> struct GravityVisitor {
+    const TreeDev &t;
+    const BoxDev &box;
+    const State &s;
+    double G, theta2;
+    double xi[D];
+    double einv_i;   // 2 / h_i
+    double phi = 0.0;
+    double acc[D];
+    double d[D], d2 = 0.0;
+    int64_t npp = 0, npn = 0;
+
+    __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_)
+    {
+#pragma unroll
+        for(int k = 0; k < D; ++k) acc[k] = 0.0;
+    }
+
+    __device__ __forceinline__ bool open(int node)
+    {
+        const double4 g = t.geo[node];
+        const double4 m = t.mom[node];
+        const double l2 = g.w * g.w;
+        d[0] = xi[0] - m.x;
+        if constexpr(D > 1) d[1] = xi[1] - m.y;
+        if constexpr(D > 2) d[2] = xi[2] - m.z;
+        min_image<D>(box, d);
+        d2 = d[0] * d[0];
+#pragma unroll
+        for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
+        return l2 > theta2 * d2;
+    }
+
+    __device__ __forceinline__ void far(int node)
+    {
+        const double m = t.mom[node].w;
+        const double rinv = 1.0 / sqrt(d2);
+        const double gm = G * m;
+        phi -= gm * rinv;
+        const double c = gm * rinv * rinv * rinv;
+#pragma unroll
+        for(int k = 0; k < D; ++k) acc[k] -= d[k] * c;
+        ++npn;
+    }
+
+    __device__ __forceinline__ void leaf(int4 r, bool open)
+    {
+        for(int j = r.x; j < r.y; ++j) {
+            double xj[D];
+#pragma unroll
+            for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
+            const double mj = s.f[F_MASS][j];
+            const double einv_j = 2.0 * s.f[F_HINV][j];
+            if(open) {
+                double rij[D];
+                const double rr = sqrt(sep2<D>(box, xi, xj, rij));
+                const double gm = G * mj;
+                phi -= gm * (soft_pot(rr, einv_i) + soft_pot(rr, einv_j)) * 0.5;
+                const double c = gm * (soft_acc(rr, einv_i) + soft_acc(rr, einv_j)) * 0.5;
+#pragma unroll
+                for(int k = 0; k < D; ++k) acc[k] -= rij[k] * c;
+                ++npp;
+            }
+        }
+    }
+};
+
+template <int D> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const bool valid = i < n;
+    GravityVisitor<D> v(t, box, s);
+    v.G = a.G;
+    v.theta2 = a.theta2;
+    v.einv_i = 0.0;
+    if(valid) {
+#pragma unroll
+        for(int k = 0; k < D; ++k) v.xi[k] = s.f[F_X + k][i];
+        v.einv_i = 2.0 * s.f[F_HINV][i];
+    } else {
+#pragma unroll
+        for(int k = 0; k < D; ++k) v.xi[k] = 0.0;
+    }
+    tree_walk(t, *t.num_nodes, valid, v);
+    if(valid) {
+        s.f[F_PHI][i] = v.phi;
+#pragma unroll
+        for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k];
+    }
+    const int64_t pp = wave_sum(v.npp), pn = wave_sum(v.npn);
+    if(wave_lane() == 0) {
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)pp);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)pn);
+    }
+}
+
+int gravity_compute(Ctx &c)
+{
+    if(!c.p.gravity_is_valid) return SPHGPU_OK;
+    GravArgs a{};
+    a.G = c.p.gravity_constant;
+    a.theta2 = c.p.gravity_theta * c.p.gravity_theta;
+    a.counters = c.counters;
+    const BoxDev box = make_box(c);
+    const int n = (int)c.n;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL(gravity_kernel<1>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL(gravity_kernel<2>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL(gravity_kernel<3>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    }
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+}  // namespace sphgpu

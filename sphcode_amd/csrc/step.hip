@@ -1,0 +1,234 @@
+// Streaming stages and global reductions: per-particle initialisation,
+// TimeStep, predict/correct kick-drift and the energy sums.  Reductions are
+// two-pass (block partials, then one block), in FP64, deterministic.
+#include "ctx.hpp"
+
+namespace sphgpu {
+
+// Solver::initialize per-particle loop (src/solver.cpp:392-404).
+template <int D> __global__ void __launch_bounds__(BLOCK) init_kernel(State s, int n, double alpha, double cs)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    s.f[F_ALPHA][i] = alpha;
+    s.f[F_BAL][i] = 1.0;
+    s.f[F_SOUND][i] = sqrt(cs * s.f[F_ENE][i]);
+}
+
+// TimeStep::calculation (src/timestep.cpp:18-38), first pass: per-block
+// min of c_force sqrt(h / |a|) over |a| > 0.
+template <int D> __global__ void __launch_bounds__(BLOCK) dt_partial_kernel(State s, int n, double cf, double *partial)
+{
+    double m = 1.7976931348623157e308;
+    for(int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        double a2 = s.f[F_A][i] * s.f[F_A][i];
+#pragma unroll
+        for(int k = 1; k < D; ++k) a2 += s.f[F_A + k][i] * s.f[F_A + k][i];
+        const double a = sqrt(a2);
+        if(a > 0.0) {
+            const double d = cf * sqrt(s.f[F_SML][i] / a);
+            m = d < m ? d : m;
+        }
+    }
+    __shared__ double sh[BLOCK / WAVE];
+    m = wave_minv(m);
+    if(wave_lane() == 0) sh[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        for(int w = 1; w < BLOCK / WAVE; ++w) m = sh[w] < m ? sh[w] : m;
+        partial[blockIdx.x] = sh[0] < m ? sh[0] : m;
+    }
+}
+
+// Second pass: dt = min(c_sound * min(h / v_sig), min dt_force).
+__global__ void dt_final_kernel(const double *partial, int nb, double csound, double *scal)
+{
+    __shared__ double sh[BLOCK / WAVE];
+    double m = 1.7976931348623157e308;
+    for(int b = threadIdx.x; b < nb; b += blockDim.x) m = partial[b] < m ? partial[b] : m;
+    m = wave_minv(m);
+    if(wave_lane() == 0) sh[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        for(int w = 1; w < (int)(blockDim.x / WAVE); ++w) m = sh[w] < m ? sh[w] : m;
+        m = sh[0] < m ? sh[0] : m;
+        const double ds = csound * scal[S_HVS];
+        scal[S_DT] = ds < m ? ds : m;
+    }
+}
+
+__global__ void min_final_kernel(const double *partial, int nb, double *scal, int slot)
+{
+    __shared__ double sh[BLOCK / WAVE];
+    double m = 1.7976931348623157e308;
+    for(int b = threadIdx.x; b < nb; b += blockDim.x) m = partial[b] < m ? partial[b] : m;
+    m = wave_minv(m);
+    if(wave_lane() == 0) sh[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        for(int w = 1; w < (int)(blockDim.x / WAVE); ++w) m = sh[w] < m ? sh[w] : m;
+        scal[slot] = sh[0] < m ? sh[0] : m;
+    }
+}
+
+// Solver::predict (src/solver.cpp:431-456), operation order kept.
+template <int D> __global__ void __launch_bounds__(BLOCK) predict_kernel(State s, int n, const double *scal, double cs, BoxDev box)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    const double dt = scal[S_DT];
+    const double hdt = 0.5 * dt;
+    double x[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) {
+        const double v = s.f[F_V + k][i], a = s.f[F_A + k][i];
+        const double vp = v + a * hdt;
+        s.f[F_VP + k][i] = vp;
+        x[k] = s.f[F_X + k][i] + vp * dt;
+        s.f[F_V + k][i] = v + a * dt;
+    }
+    const double u = s.f[F_ENE][i], du = s.f[F_DENE][i];
+    s.f[F_ENEP][i] = u + du * hdt;
+    const double un = u + du * dt;
+    s.f[F_ENE][i] = un;
+    s.f[F_SOUND][i] = sqrt(cs * un);
+    wrap<D>(box, x);
+#pragma unroll
+    for(int k = 0; k < D; ++k) s.f[F_X + k][i] = x[k];
+}
+
+// Solver::correct (src/solver.cpp:458-474).
+template <int D> __global__ void __launch_bounds__(BLOCK) correct_kernel(State s, int n, const double *scal, double cs)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    const double hdt = 0.5 * scal[S_DT];
+#pragma unroll
+    for(int k = 0; k < D; ++k) s.f[F_V + k][i] = s.f[F_VP + k][i] + s.f[F_A + k][i] * hdt;
+    const double u = s.f[F_ENEP][i] + s.f[F_DENE][i] * hdt;
+    s.f[F_ENE][i] = u;
+    s.f[F_SOUND][i] = sqrt(cs * u);
+}
+
+// Energy sums of Output::output_energy (src/output.cpp:66-90) and momentum.
+constexpr int NE = 7;  // kinetic, thermal, potential, momentum x3
+template <int D> __global__ void __launch_bounds__(BLOCK) energy_partial_kernel(State s, int n, double *partial)
+{
+    double e[NE] = {0, 0, 0, 0, 0, 0, 0};
+    for(int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const double m = s.f[F_MASS][i];
+        double v2 = 0.0;
+#pragma unroll
+        for(int k = 0; k < D; ++k) {
+            const double v = s.f[F_V + k][i];
+            v2 += v * v;
+            e[3 + k] += m * v;
+        }
+        e[0] += 0.5 * m * v2;
+        e[1] += m * s.f[F_ENE][i];
+        e[2] += 0.5 * m * s.f[F_PHI][i];
+    }
+    __shared__ double sh[NE][BLOCK / WAVE];
+#pragma unroll
+    for(int k = 0; k < NE; ++k) {
+        const double v = wave_sum(e[k]);
+        if(wave_lane() == 0) sh[k][threadIdx.x / WAVE] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < NE) {
+        double t = 0.0;
+        for(int w = 0; w < BLOCK / WAVE; ++w) t += sh[threadIdx.x][w];
+        partial[blockIdx.x * NE + threadIdx.x] = t;
+    }
+}
+
+__global__ void energy_final_kernel(const double *partial, int nb, double *out)
+{
+    if(threadIdx.x >= NE) return;
+    double t = 0.0;
+    for(int b = 0; b < nb; ++b) t += partial[b * NE + threadIdx.x];
+    out[threadIdx.x] = t;
+}
+
+// ------------------------------------------------------------------ host
+BoxDev make_box(const Ctx &c)
+{
+    BoxDev box{};
+    box.on = c.p.periodic_is_valid;
+    for(int k = 0; k < 3; ++k) {
+        box.lo[k] = c.p.range_min[k];
+        box.hi[k] = c.p.range_max[k];
+        box.len[k] = c.p.range_max[k] - c.p.range_min[k];
+    }
+    return box;
+}
+
+#define SPHGPU_DISPATCH_D(c, KERNEL, grid, ...)                                                     \
+    switch((c).D) {                                                                                 \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(grid), dim3(BLOCK), 0, (c).stream, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(grid), dim3(BLOCK), 0, (c).stream, __VA_ARGS__); break; \
+    default: hipLaunchKernelGGL(KERNEL<3>, dim3(grid), dim3(BLOCK), 0, (c).stream, __VA_ARGS__); break; \
+    }
+
+int step_init_fields(Ctx &c)
+{
+    const double cs = c.p.gamma * (c.p.gamma - 1.0);
+    SPHGPU_DISPATCH_D(c, init_kernel, grid_for(c.n), c.s, (int)c.n, c.p.av_alpha, cs);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int reduce_min_partials(Ctx &c, int nblocks, int slot)
+{
+    hipLaunchKernelGGL(min_final_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.partial, nblocks, c.scal, slot);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int step_timestep(Ctx &c)
+{
+    const int nb = std::min(grid_for(c.n), 1024);
+    SPHGPU_DISPATCH_D(c, dt_partial_kernel, nb, c.s, (int)c.n, c.p.cfl_force, c.partial);
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.partial, nb, c.p.cfl_sound, c.scal);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int step_predict(Ctx &c)
+{
+    const double cs = c.p.gamma * (c.p.gamma - 1.0);
+    const BoxDev box = make_box(c);
+    SPHGPU_DISPATCH_D(c, predict_kernel, grid_for(c.n), c.s, (int)c.n, c.scal, cs, box);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int step_correct(Ctx &c)
+{
+    const double cs = c.p.gamma * (c.p.gamma - 1.0);
+    SPHGPU_DISPATCH_D(c, correct_kernel, grid_for(c.n), c.s, (int)c.n, c.scal, cs);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int step_energy(Ctx &c, double *out)
+{
+    const int nb = std::min(grid_for(c.n), 1024);
+    SPHGPU_DISPATCH_D(c, energy_partial_kernel, nb, c.s, (int)c.n, c.partial);
+    double *dev_out = c.partial + 1024 * NE;
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.partial, nb, dev_out);
+    SPHGPU_HIP(hipGetLastError());
+    double host[NE];
+    SPHGPU_HIP(hipMemcpyAsync(host, dev_out, sizeof(host), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    out[0] = host[0];
+    out[1] = host[1];
+    out[2] = host[2];
+    out[3] = host[0] + host[1] + host[2];
+    for(int k = 0; k < c.D; ++k) out[4 + k] = host[3 + k];
+    return SPHGPU_OK;
+}
+
+}  // namespace sphgpu

@@ -1,0 +1,469 @@
+// Octree build on the device (replaces BHTree::make, src/bhtree.cpp:55-107,
+// and BHTree::set_kernel, src/bhtree.cpp:109-112, 206-232).
+//
+// The reference builds a pointer octree top-down, serially: the root cube
+// is the bounding box (centre, largest extent) or the periodic box
+// (src/bhtree.cpp:19-31, 59-94); a particle goes to child slot
+// sum_k [x_k > c_k] << k (src/bhtree.cpp:168-175); a child's centre is the
+// parent's +-edge/4 and its edge half the parent's (src/bhtree.cpp:188-196);
+// a child at level L is split when it holds more than leafParticleNumber
+// particles and L-1 < maxTreeLevel (src/bhtree.cpp:154-158; root level 1).
+//
+// Here every particle replays that descent once, with the same FP64
+// comparisons and centre updates, to get a key of maxTreeLevel D-bit slot
+// groups (most significant = first split).  Sorting the keys puts each
+// cell's particles in one contiguous range and the cells in depth-first
+// pre-order, with children in slot order -- the reference's visiting order.
+// The cells that exist, and which of them are leaves, follow from counts of
+// equal key prefixes; a node starting at particle b exists at level L iff
+// b begins a level-L cell whose parent holds more than leafParticleNumber
+// particles.  Each particle emits the nodes that start at it, so an
+// exclusive scan of those counts is the pre-order numbering.  Mass moments
+// and the largest h per subtree are accumulated bottom-up with one
+// arrival counter per node.  The root keeps zero mass and centre of mass,
+// as the reference never accumulates them (include/bhtree.hpp:46-53).
+#include <cstring>
+
+#include "ctx.hpp"
+
+#include <rocprim/rocprim.hpp>
+
+namespace sphgpu {
+
+// ------------------------------------------------------------ bounding box
+template <int D> __global__ void __launch_bounds__(BLOCK) bbox_partial(State s, int n, double *partial)
+{
+    double lo[D], hi[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) { lo[k] = 1.7976931348623157e308; hi[k] = -1.7976931348623157e308; }
+    for(int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
+#pragma unroll
+        for(int k = 0; k < D; ++k) {
+            const double x = s.f[F_X + k][i];
+            lo[k] = x < lo[k] ? x : lo[k];
+            hi[k] = x > hi[k] ? x : hi[k];
+        }
+    __shared__ double sh[2 * D][BLOCK / WAVE];
+#pragma unroll
+    for(int k = 0; k < D; ++k) {
+        lo[k] = wave_minv(lo[k]);
+        hi[k] = wave_maxv(hi[k]);
+    }
+    const int w = threadIdx.x / WAVE;
+    if(wave_lane() == 0)
+#pragma unroll
+        for(int k = 0; k < D; ++k) { sh[k][w] = lo[k]; sh[D + k][w] = hi[k]; }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+#pragma unroll
+        for(int k = 0; k < D; ++k) {
+            double a = sh[k][0], b = sh[D + k][0];
+            for(int j = 1; j < BLOCK / WAVE; ++j) {
+                a = sh[k][j] < a ? sh[k][j] : a;
+                b = sh[D + k][j] > b ? sh[D + k][j] : b;
+            }
+            partial[blockIdx.x * 2 * D + k] = a;
+            partial[blockIdx.x * 2 * D + D + k] = b;
+        }
+    }
+}
+
+// Root cube: centre (max+min)/2, edge = largest extent (src/bhtree.cpp:80-94).
+template <int D> __global__ void bbox_final(const double *partial, int nblocks, double *scal)
+{
+#pragma clang fp contract(off)
+    if(threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[D], hi[D];
+    for(int k = 0; k < D; ++k) { lo[k] = partial[k]; hi[k] = partial[D + k]; }
+    for(int b = 1; b < nblocks; ++b)
+        for(int k = 0; k < D; ++k) {
+            const double a = partial[b * 2 * D + k], c = partial[b * 2 * D + D + k];
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = c > hi[k] ? c : hi[k];
+        }
+    double edge = 0.0;
+    for(int k = 0; k < D; ++k) {
+        scal[S_ROOT_CX + k] = (hi[k] + lo[k]) * 0.5;
+        const double r = hi[k] - lo[k];
+        if(edge < r) edge = r;
+    }
+    scal[S_ROOT_EDGE] = edge;
+}
+
+// ------------------------------------------------------------------- keys
+// The reference's descent, one level per slot group (src/bhtree.cpp:163-204).
+template <int D> __global__ void __launch_bounds__(BLOCK) keys_kernel(State s, int n, const double *scal, int levels, uint64_t *keys, int *idx)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    double c[D], x[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) { c[k] = scal[S_ROOT_CX + k]; x[k] = s.f[F_X + k][i]; }
+    double e = scal[S_ROOT_EDGE];
+    uint64_t key = 0;
+    for(int L = 0; L < levels; ++L) {
+        uint64_t slot = 0;
+#pragma unroll
+        for(int k = 0; k < D; ++k) {
+            const bool up = x[k] > c[k];
+            slot |= (uint64_t)up << k;
+            c[k] = c[k] + (up ? 1.0 : -1.0) * e * 0.25;
+        }
+        e = e * 0.5;
+        key = (key << D) | slot;
+    }
+    keys[i] = key;
+    idx[i] = i;
+}
+
+// --------------------------------------------------------------- reorder
+struct Reorder {
+    const double *src[NF];
+    double *dst[NF];
+    const int *isrc[2];
+    int *idst[2];
+};
+
+__global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *perm, int n, int nfields)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    const int j = perm[i];
+    for(int f = 0; f < nfields; ++f) r.dst[f][i] = r.src[f][j];
+    r.idst[0][i] = r.isrc[0][j];
+    r.idst[1][i] = r.isrc[1][j];
+}
+
+// ----------------------------------------------------------------- delta
+// delta[i] = number of equal leading slot groups of keys i-1 and i: the two
+// particles share the cells of levels 1 .. delta+1.
+__global__ void __launch_bounds__(BLOCK) delta_kernel(const uint64_t *k, int n, int G, int D, uint8_t *delta)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    if(i == 0) { delta[0] = 0; return; }
+    const uint64_t x = k[i - 1] ^ k[i];
+    if(x == 0) { delta[i] = (uint8_t)G; return; }
+    const int lz = __clzll((long long)x) - (64 - G * D);
+    delta[i] = (uint8_t)(lz / D);
+}
+
+struct TreeShape {
+    int n, G, D, leaf, max_level;
+};
+
+// First level at which particle b begins a cell.
+__device__ __forceinline__ int first_level(const uint8_t *delta, int b) { return b == 0 ? 2 : (int)delta[b] + 2; }
+
+// Size of the level-L cell that begins at b, or leaf+1 if it is larger
+// than leaf.  A later particle j begins a new level-L cell iff
+// delta[j] < L-1.
+__device__ __forceinline__ int cell_count_bounded(const uint8_t *delta, const TreeShape &ts, int b, int L)
+{
+    for(int j = b + 1; j <= b + ts.leaf; ++j) {
+        if(j >= ts.n) return ts.n - b;
+        if((int)delta[j] < L - 1) return j - b;
+    }
+    return ts.leaf + 1;
+}
+
+// Does the level-P cell containing b-1 and b hold more than leaf particles?
+__device__ __forceinline__ bool parent_split(const uint8_t *delta, const TreeShape &ts, int b, int P)
+{
+    int cnt = 1;
+    int j = b;
+    while(cnt <= ts.leaf && j > 0 && (int)delta[j] >= P - 1) { --j; ++cnt; }
+    int k = b + 1;
+    while(cnt <= ts.leaf && k < ts.n && (int)delta[k] >= P - 1) { ++k; ++cnt; }
+    return cnt > ts.leaf;
+}
+
+// Number of nodes that begin at particle b (all at consecutive levels).
+__device__ int nodes_at(const uint8_t *delta, const TreeShape &ts, int b, int &lvl0)
+{
+    if(b > 0 && (int)delta[b] >= ts.G) return 0;  // same deepest cell as b-1
+    lvl0 = first_level(delta, b);
+    if(lvl0 > 2 && !parent_split(delta, ts, b, lvl0 - 1)) return 0;
+    int k = 0;
+    for(int L = lvl0;; ++L) {
+        ++k;
+        const int cnt = cell_count_bounded(delta, ts, b, L);
+        if(!(cnt > ts.leaf && L <= ts.max_level)) break;
+    }
+    return k;
+}
+
+__global__ void __launch_bounds__(BLOCK) count_nodes_kernel(const uint8_t *delta, TreeShape ts, int *nn)
+{
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if(b >= ts.n) return;
+    int lvl0 = 0;
+    nn[b] = nodes_at(delta, ts, b, lvl0);
+}
+
+// End (exclusive) of the level-L cell that begins at b: first particle
+// whose key differs in the first L-1 slot groups.
+__device__ __forceinline__ int cell_end(const uint64_t *keys, const TreeShape &ts, int b, int L)
+{
+    const int shift = (ts.G - (L - 1)) * ts.D;
+    const uint64_t pre = shift >= 64 ? 0 : keys[b] >> shift;
+    int lo = b + 1, hi = ts.n;
+    while(lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t pm = shift >= 64 ? 0 : keys[mid] >> shift;
+        if(pm > pre) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+template <int D>
+__global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys, const uint8_t *delta, TreeShape ts, const int *nn,
+                                                        const int *noff, const double *scal, TreeDev t, int64_t node_cap,
+                                                        int64_t *counters)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if(b >= ts.n) return;
+    const int total = 1 + noff[ts.n - 1] + nn[ts.n - 1];
+    if(b == 0) {
+        *t.num_nodes = total <= node_cap ? total : (int)node_cap;
+        if(total > node_cap) atomicAdd((unsigned long long *)&counters[C_TREE_ERR], 1ull);
+        t.geo[0] = make_double4(scal[S_ROOT_CX], D > 1 ? scal[S_ROOT_CY] : 0.0, D > 2 ? scal[S_ROOT_CZ] : 0.0, scal[S_ROOT_EDGE]);
+        t.rng[0] = make_int4(0, ts.n, total, 1);
+    }
+    const int cnt = nn[b];
+    if(cnt == 0) return;
+    int lvl0 = 0;
+    nodes_at(delta, ts, b, lvl0);
+    const int base = 1 + noff[b];
+    const uint64_t key = keys[b];
+    const uint64_t mask = (1ull << D) - 1ull;
+    double c[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) c[k] = scal[S_ROOT_CX + k];
+    double e = scal[S_ROOT_EDGE];
+    // replay the descent down to level lvl0 - 1
+    int g = 0;
+    for(; g < lvl0 - 2; ++g) {
+        const uint64_t slot = (key >> ((ts.G - 1 - g) * D)) & mask;
+#pragma unroll
+        for(int k = 0; k < D; ++k) c[k] = c[k] + (((slot >> k) & 1) ? 1.0 : -1.0) * e * 0.25;
+        e = e * 0.5;
+    }
+    for(int j = 0; j < cnt; ++j) {
+        const int L = lvl0 + j;
+        // descend one more level: cell of level L
+        const uint64_t slot = (key >> ((ts.G - 1 - g) * D)) & mask;
+#pragma unroll
+        for(int k = 0; k < D; ++k) c[k] = c[k] + (((slot >> k) & 1) ? 1.0 : -1.0) * e * 0.25;
+        e = e * 0.5;
+        ++g;
+        const int node = base + j;
+        const bool leaf = j == cnt - 1;
+        int end;
+        if(leaf && L <= ts.max_level) end = b + cell_count_bounded(delta, ts, b, L);
+        else end = cell_end(keys, ts, b, L);
+        const int skip = end < ts.n ? 1 + noff[end] : total;
+        if(node < node_cap) {
+            t.geo[node] = make_double4(c[0], D > 1 ? c[D > 1 ? 1 : 0] : 0.0, D > 2 ? c[D > 2 ? 2 : 0] : 0.0, e);
+            t.rng[node] = make_int4(b, end, skip, L | (leaf ? 256 : 0));
+        }
+    }
+}
+
+// Parent links and child counts; resets arrival counters.
+__global__ void __launch_bounds__(BLOCK) link_kernel(TreeDev t, int *parent, int *nchild, int *arrive)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int total = *t.num_nodes;
+    if(p >= total) return;
+    if(p == 0) parent[0] = -1;
+    const int4 r = t.rng[p];
+    arrive[p] = 0;
+    if(r.w & 256) { nchild[p] = 0; return; }
+    int cnt = 0;
+    for(int c = p + 1; c < r.z && c < total && cnt < 8; c = t.rng[c].z) {
+        parent[c] = p;
+        ++cnt;
+    }
+    nchild[p] = cnt;
+}
+
+__global__ void __launch_bounds__(BLOCK) reset_parent_kernel(int64_t cap, int *parent)
+{
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if(p < cap) parent[p] = -1;
+}
+
+__global__ void __launch_bounds__(BLOCK) reset_arrive_kernel(const int *num_nodes, int *arrive)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if(p < *num_nodes) arrive[p] = 0;
+}
+
+__device__ __forceinline__ bool last_arrival(int *arrive, const int *nchild, int q)
+{
+    const int old = __hip_atomic_fetch_add(&arrive[q], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return old == nchild[q] - 1;
+}
+
+// Mass and centre of mass, leaves first then up the tree.
+template <int D>
+__global__ void __launch_bounds__(BLOCK) moments_kernel(State s, TreeDev t, const int *parent, const int *nchild, int *arrive, double4 *msum)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int total = *t.num_nodes;
+    if(p >= total) return;
+    const int4 r = t.rng[p];
+    if(!(r.w & 256)) return;
+    double m = 0.0, mx[3] = {0.0, 0.0, 0.0};
+    for(int i = r.x; i < r.y; ++i) {
+        const double mi = s.f[F_MASS][i];
+        m += mi;
+#pragma unroll
+        for(int k = 0; k < D; ++k) mx[k] += s.f[F_X + k][i] * mi;
+    }
+    msum[p] = make_double4(mx[0], mx[1], mx[2], m);
+    t.mom[p] = make_double4(mx[0] / m, mx[1] / m, mx[2] / m, m);
+    int q = parent[p];
+    for(int depth = 0; q >= 0 && q < total && depth < 64; ++depth) {
+        if(!last_arrival(arrive, nchild, q)) return;
+        const int4 rq = t.rng[q];
+        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+        int guard = 0;
+        for(int c = q + 1; c < rq.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
+            const double4 v = msum[c];
+            acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+        }
+        msum[q] = acc;
+        if(q == 0) t.mom[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // root: never accumulated
+        else t.mom[q] = make_double4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
+        q = parent[q];
+    }
+}
+
+// Largest smoothing length per subtree (BHNode::set_kernel).
+__global__ void __launch_bounds__(BLOCK) kernel_size_kernel(State s, TreeDev t, const int *parent, const int *nchild, int *arrive)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int total = *t.num_nodes;
+    if(p >= total) return;
+    const int4 r = t.rng[p];
+    if(!(r.w & 256)) return;
+    double h = 0.0;
+    for(int i = r.x; i < r.y; ++i) {
+        const double hi = s.f[F_SML][i];
+        h = hi > h ? hi : h;
+    }
+    t.ks[p] = h;
+    int q = parent[p];
+    for(int depth = 0; q >= 0 && q < total && depth < 64; ++depth) {
+        if(!last_arrival(arrive, nchild, q)) return;
+        const int4 rq = t.rng[q];
+        double m = 0.0;
+        int guard = 0;
+        for(int c = q + 1; c < rq.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
+            const double v = t.ks[c];
+            m = v > m ? v : m;
+        }
+        t.ks[q] = m;
+        q = parent[q];
+    }
+}
+
+// ------------------------------------------------------------------ host
+int tree_alloc(Ctx &c)
+{
+    const int64_t n = c.cap_n;
+    c.node_cap = 5 * n + 1;  // BHTree::resize: 5 nodes per particle (+ root)
+    SPHGPU_HIP(hipMalloc(&c.keys, n * sizeof(uint64_t)));
+    SPHGPU_HIP(hipMalloc(&c.keys_sorted, n * sizeof(uint64_t)));
+    SPHGPU_HIP(hipMalloc(&c.idx, n * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.perm, n * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.delta, n));
+    SPHGPU_HIP(hipMalloc(&c.nn, n * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.noff, n * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.t.geo, c.node_cap * sizeof(double4)));
+    SPHGPU_HIP(hipMalloc(&c.t.mom, c.node_cap * sizeof(double4)));
+    SPHGPU_HIP(hipMalloc(&c.t.ks, c.node_cap * sizeof(double)));
+    SPHGPU_HIP(hipMalloc(&c.t.rng, c.node_cap * sizeof(int4)));
+    SPHGPU_HIP(hipMalloc(&c.t.num_nodes, sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.parent, c.node_cap * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.nchild, c.node_cap * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.arrive, c.node_cap * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.msum, c.node_cap * sizeof(double4)));
+    size_t bytes = 0;
+    const unsigned key_bits = (unsigned)(c.p.tree_max_level * c.D);
+    SPHGPU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c.keys, c.keys_sorted, c.idx, c.perm, (unsigned)n, 0, key_bits, c.stream));
+    c.sort_tmp_bytes = bytes;
+    SPHGPU_HIP(hipMalloc(&c.sort_tmp, bytes));
+    bytes = 0;
+    SPHGPU_HIP(rocprim::exclusive_scan(nullptr, bytes, c.nn, c.noff, 0, (size_t)n, rocprim::plus<int>(), c.stream));
+    c.scan_tmp_bytes = bytes;
+    SPHGPU_HIP(hipMalloc(&c.scan_tmp, bytes));
+    return SPHGPU_OK;
+}
+
+template <int D> static int tree_build_d(Ctx &c)
+{
+    const int n = (int)c.n;
+    const int G = c.p.tree_max_level;
+    // root geometry
+    if(!c.p.periodic_is_valid) {
+        const int nb = std::min(grid_for(n), 1024);
+        hipLaunchKernelGGL(bbox_partial<D>, dim3(nb), dim3(BLOCK), 0, c.stream, c.s, n, c.partial);
+        hipLaunchKernelGGL(bbox_final<D>, dim3(1), dim3(WAVE), 0, c.stream, c.partial, nb, c.scal);
+    }
+    hipLaunchKernelGGL(keys_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, n, c.scal, G, c.keys, c.idx);
+    size_t bytes = c.sort_tmp_bytes;
+    SPHGPU_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, c.keys, c.keys_sorted, c.idx, c.perm, (unsigned)n, 0, (unsigned)(G * D), c.stream));
+    // gather the state into key order
+    Reorder r{};
+    int nf = 0;
+    for(int f = 0; f < NF; ++f) {
+        if(!c.s.f[f]) continue;
+        r.src[nf] = c.s.f[f];
+        r.dst[nf] = c.alt.f[f];
+        ++nf;
+    }
+    r.isrc[0] = c.s.id; r.idst[0] = c.alt.id;
+    r.isrc[1] = c.s.nb; r.idst[1] = c.alt.nb;
+    hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, r, c.perm, n, nf);
+    std::swap(c.s, c.alt);
+
+    const TreeShape ts{n, G, D, c.p.tree_leaf_particle_num, c.p.tree_max_level};
+    hipLaunchKernelGGL(delta_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, n, G, D, c.delta);
+    hipLaunchKernelGGL(count_nodes_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.delta, ts, c.nn);
+    bytes = c.scan_tmp_bytes;
+    SPHGPU_HIP(rocprim::exclusive_scan(c.scan_tmp, bytes, c.nn, c.noff, 0, (size_t)n, rocprim::plus<int>(), c.stream));
+    hipLaunchKernelGGL(emit_nodes_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, c.delta, ts, c.nn, c.noff,
+                       c.scal, c.t, c.node_cap, c.counters);
+    const int ngrid = grid_for(c.node_cap);
+    hipLaunchKernelGGL(reset_parent_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.node_cap, c.parent);
+    hipLaunchKernelGGL(link_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.nchild, c.arrive);
+    hipLaunchKernelGGL(moments_kernel<D>, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.s, c.t, c.parent, c.nchild, c.arrive, c.msum);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int tree_build(Ctx &c)
+{
+    switch(c.D) {
+    case 1: return tree_build_d<1>(c);
+    case 2: return tree_build_d<2>(c);
+    default: return tree_build_d<3>(c);
+    }
+}
+
+int tree_set_kernel(Ctx &c)
+{
+    const int ngrid = grid_for(c.node_cap);
+    hipLaunchKernelGGL(reset_arrive_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t.num_nodes, c.arrive);
+    hipLaunchKernelGGL(kernel_size_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.s, c.t, c.parent, c.nchild, c.arrive);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+}  // namespace sphgpu

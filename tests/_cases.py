@@ -1,0 +1,89 @@
+"""Small seeded workloads shared by the oracle and GPU parity tests.
+
+Each case returns (params, fields) from sphcode_amd.ics at a size the CPU
+oracle finishes in about a second.  Every (DIM, SPHType, kernel) pair the
+benchmark configurations use appears at least once, plus the second-order
+and first-order GSPH variants and artificial conductivity.
+"""
+from __future__ import annotations
+
+from sphcode_amd import _abi, ics
+
+
+def sod_ssph():
+    return ics.sod(400, 50)
+
+
+def sod_disph():
+    p, f = ics.sod(400, 50)
+    p.sph_type = _abi.DISPH
+    return p, f
+
+
+def sod_gsph2():
+    p, f = ics.sod(400, 50)
+    p.sph_type = _abi.GSPH
+    p.gsph_is_2nd_order = 1
+    return p, f
+
+
+def sod_gsph1():
+    p, f = ics.sod(400, 50)
+    p.sph_type = _abi.GSPH
+    p.gsph_is_2nd_order = 0
+    return p, f
+
+
+def sod_ac_tdav():
+    p, f = ics.sod(400, 50)
+    p.ac_is_valid = 1
+    p.ac_alpha = 1.0
+    p.av_use_time_dependent = 1
+    return p, f
+
+
+def khi_disph():
+    return ics.khi(n_low=32, n_high=45, jitter=0.05)
+
+
+def khi_ssph():
+    p, f = ics.khi(n_low=32, n_high=45, jitter=0.05, sph_type=_abi.SSPH)
+    p.kernel = _abi.CUBIC_SPLINE
+    p.neighbor_number = 32
+    return p, f
+
+
+def evrard_disph():
+    return ics.evrard(4096, leaf=32)
+
+
+def evrard_ssph():
+    p, f = ics.evrard(2048, leaf=8)
+    p.sph_type = _abi.SSPH
+    p.kernel = _abi.CUBIC_SPLINE
+    p.neighbor_number = 50
+    p.av_use_time_dependent = 0
+    return p, f
+
+
+def rollout_ssph():
+    return ics.rollout(n_side=12)
+
+
+def sedov_gsph():
+    return ics.sedov(n_side=12, n_hot=8)
+
+
+CASES = {
+    "sod_ssph": sod_ssph,
+    "sod_disph": sod_disph,
+    "sod_gsph2": sod_gsph2,
+    "sod_gsph1": sod_gsph1,
+    "sod_ac_tdav": sod_ac_tdav,
+    "khi_disph": khi_disph,
+    "khi_ssph": khi_ssph,
+    "evrard_disph": evrard_disph,
+    "evrard_ssph": evrard_ssph,
+    "rollout_ssph": rollout_ssph,
+    "sedov_gsph": sedov_gsph,
+}

@@ -1,0 +1,183 @@
+"""FP64 parity of the HIP path against the CPU oracle on the same inputs.
+
+For every case of tests/_cases.py both engines receive the same particles
+and run Solver::initialize (tree, pre-interaction, fluid force, gravity),
+then three full steps.  Bar (FP64 path):
+  * neighbour sets: identical as sets, per particle, for the gather
+    candidates (r^2 < (1.2 h_guess)^2) and for the force pairs
+    (r < max(h_i, h_j), r != 0); neighbour counts identical;
+  * h, rho, P, grad-h, sound: relative error <= 1e-9 after initialize
+    (only summation order differs);
+  * acceleration, du/dt: error <= 1e-8 of the field's RMS; phi <= 1e-9
+    relative; Balsara switch and alpha <= 1e-8 absolute;
+  * after three steps (positions have then drifted by round-off):
+    relative error <= 1e-6 on h, rho, P, positions, velocities.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _cases import CASES
+from _oracle import OracleEngine
+
+pytestmark = pytest.mark.gpu
+
+SCALARS_TIGHT = ("sml", "dens", "pres", "sound")
+
+
+def _engines(case):
+    from sphcode_amd import GPUEngine
+    p, f = CASES[case]()
+    ref = OracleEngine(p)
+    gpu = GPUEngine(p)
+    for e in (ref, gpu):
+        e.set_record_sets(True)
+        e.upload(f)
+    return p, f, ref, gpu
+
+
+def _rel(a, b):
+    scale = np.maximum(np.abs(b), 1e-300)
+    return np.max(np.abs(a - b) / scale)
+
+
+def _rms_err(a, b):
+    rms = np.sqrt(np.mean(b * b)) + 1e-300
+    return np.max(np.abs(a - b)) / rms
+
+
+def _min_image(d, p):
+    if not p.periodic_is_valid:
+        return d
+    L = np.array([p.range_max[k] - p.range_min[k] for k in range(p.dim)])
+    return d - L * np.round(d / L)
+
+
+def _check_sets(ref, gpu, case, p=None):
+    """Gather sets must be identical.  Force sets must be identical except
+    for pairs within round-off of the cut r = max(h_i, h_j): on a perfect
+    lattice (the 1D tube) pairs sit exactly on the cut and h, a Newton
+    result summed in a different order, decides them at the last bit."""
+    rs = ref.neighbor_sets(0)
+    gs = gpu.neighbor_sets(0)
+    bad = [i for i in range(len(rs)) if not np.array_equal(rs[i], gs[i])]
+    assert not bad, f"{case}: gather sets differ for {len(bad)} particles, first {bad[:5]}"
+    rs = ref.neighbor_sets(1)
+    gs = gpu.neighbor_sets(1)
+    st = ref.download(("pos", "sml"))
+    x, h = st["pos"], st["sml"]
+    n_tie = 0
+    for i in range(len(rs)):
+        if np.array_equal(rs[i], gs[i]):
+            continue
+        diff = np.setxor1d(rs[i], gs[i])
+        d = _min_image(x[diff] - x[i], p) if p is not None else x[diff] - x[i]
+        r = np.sqrt((d * d).sum(1))
+        cut = np.maximum(h[i], h[diff])
+        rel = np.abs(r / cut - 1.0)
+        assert np.all(rel < 1e-12), f"{case}: force set of {i} differs away from the cut ({rel.max():.3e})"
+        n_tie += diff.size
+    print(f"[sets] {case}: force-set members decided at round-off of the cut: {n_tie}", flush=True)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_initialize_parity(gpu, case):
+    p, f, ref, gpu_e = _engines(case)
+    print(f"[parity] case={case} n={len(f['mass'])} dim={p.dim}", flush=True)
+    ref.initialize()
+    gpu_e.initialize()
+    _check_sets(ref, gpu_e, case, p)
+    a = ref.download()
+    b = gpu_e.download()
+    assert np.array_equal(a["neighbor"], b["neighbor"]), f"{case}: neighbour counts differ"
+    for k in SCALARS_TIGHT:
+        assert _rel(b[k], a[k]) <= 1e-9, f"{case}: {k} rel err {_rel(b[k], a[k]):.3e}"
+    if p.sph_type != 2:
+        assert _rel(b["gradh"], a["gradh"]) <= 1e-9, f"{case}: gradh"
+        assert np.max(np.abs(b["balsara"] - a["balsara"])) <= 1e-8, f"{case}: balsara"
+        assert np.max(np.abs(b["alpha"] - a["alpha"])) <= 1e-8, f"{case}: alpha"
+    assert _rms_err(b["acc"], a["acc"]) <= 1e-8, f"{case}: acc err {_rms_err(b['acc'], a['acc']):.3e}"
+    assert _rms_err(b["dene"], a["dene"]) <= 1e-8, f"{case}: dene err {_rms_err(b['dene'], a['dene']):.3e}"
+    if p.gravity_is_valid:
+        assert _rel(b["phi"], a["phi"]) <= 1e-9, f"{case}: phi rel err {_rel(b['phi'], a['phi']):.3e}"
+    if p.sph_type == 2 and p.gsph_is_2nd_order:
+        for k in ("grad_density", "grad_pressure", "grad_velocity"):
+            assert _rms_err(b[k], a[k]) <= 1e-8, f"{case}: {k}"
+    # energies and scalars
+    ta, tb = ref.time(), gpu_e.time()
+    assert abs(tb[2] - ta[2]) <= 1e-12 * abs(ta[2]), f"{case}: h/v_sig {tb[2]} vs {ta[2]}"
+    ea, eb = ref.energy(), gpu_e.energy()
+    assert np.allclose(eb[:4], ea[:4], rtol=1e-10, atol=1e-14), f"{case}: energy {eb} vs {ea}"
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_three_steps(gpu, case):
+    p, f, ref, gpu_e = _engines(case)
+    print(f"[steps] case={case} n={len(f['mass'])}", flush=True)
+    ref.initialize()
+    gpu_e.initialize()
+    for _ in range(3):
+        ref.integrate()
+        gpu_e.integrate()
+        ta, tb = ref.time(), gpu_e.time()
+        assert abs(tb[1] - ta[1]) <= 1e-9 * ta[1], f"{case}: dt {tb[1]} vs {ta[1]}"
+    a = ref.download()
+    b = gpu_e.download()
+    for k in ("sml", "dens", "pres"):
+        assert _rel(b[k], a[k]) <= 1e-6, f"{case}: {k} rel err {_rel(b[k], a[k]):.3e}"
+    for k in ("pos", "vel"):
+        assert _rms_err(b[k], a[k]) <= 1e-6, f"{case}: {k} err {_rms_err(b[k], a[k]):.3e}"
+    assert _rms_err(b["acc"], a["acc"]) <= 1e-5, f"{case}: acc err {_rms_err(b['acc'], a['acc']):.3e}"
+
+
+def test_midrun_state_parity(gpu):
+    """A state with nonzero velocities and evolved alpha, dt > 0 (TDAV)."""
+    from sphcode_amd import GPUEngine
+    p, f = CASES["evrard_disph"]()
+    pre = OracleEngine(p)
+    pre.upload(f)
+    pre.initialize()
+    for _ in range(5):
+        pre.integrate()
+    t, dt, _ = pre.time()
+    state = pre.download()
+    ref = OracleEngine(p)
+    gpu_e = GPUEngine(p)
+    for e in (ref, gpu_e):
+        e.set_record_sets(True)
+        e.upload(state)
+        e.set_time(t, dt)
+        e.make_tree()
+        e.pre_interaction()
+        e.fluid_force()
+        e.gravity_force()
+    _check_sets(ref, gpu_e, "midrun", p)
+    a, b = ref.download(), gpu_e.download()
+    assert np.array_equal(a["neighbor"], b["neighbor"])
+    for k in SCALARS_TIGHT + ("gradh",):
+        assert _rel(b[k], a[k]) <= 1e-9, k
+    assert np.max(np.abs(b["alpha"] - a["alpha"])) <= 1e-8
+    assert np.max(np.abs(b["balsara"] - a["balsara"])) <= 1e-8
+    assert _rms_err(b["acc"], a["acc"]) <= 1e-8
+    assert _rms_err(b["dene"], a["dene"]) <= 1e-8
+
+
+def test_tree_matches_reference_cells(gpu):
+    """Node geometry and counts equal the oracle's pointer octree."""
+    from sphcode_amd import GPUEngine
+    for case in ("evrard_disph", "khi_disph", "sod_ssph"):
+        p, f = CASES[case]()
+        ref = OracleEngine(p)
+        g = GPUEngine(p)
+        for e in (ref, g):
+            e.upload(f)
+            e.make_tree()
+        a, b = ref.tree_info(), g.tree_info()
+        assert a["num_nodes"] == b["num_nodes"], case
+        assert np.array_equal(a["level"], b["level"]), case
+        assert np.array_equal(a["count"][1:], b["count"][1:]), case
+        assert np.array_equal(a["edge"], b["edge"]), case
+        assert np.array_equal(a["center"], b["center"]), case
+        assert np.allclose(a["mass"], b["mass"], rtol=1e-12, atol=0), case
+        assert np.allclose(a["com"][1:], b["com"][1:], rtol=1e-11, atol=1e-13), case
