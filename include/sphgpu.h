@@ -153,6 +153,7 @@ typedef struct sphgpu_stats {
     int64_t max_list_length;      /* longest neighbour list seen              */
     int64_t tree_nodes;           /* nodes of the last tree                   */
     int64_t tree_max_level;       /* deepest level of the last tree           */
+    int64_t gravity_opened;       /* cells opened by the gravity walk         */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

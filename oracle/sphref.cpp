@@ -248,7 +248,7 @@ struct Tree {
 struct Stats {
     int64_t steps = 0, gather_pairs = 0, force_pairs = 0, gravity_pp = 0, gravity_pn = 0;
     int64_t newton_failures = 0, newton_iterations = 0, search_truncations = 0;
-    int64_t list_overflows = 0, max_list_length = 0, tree_nodes = 0, tree_max_level = 0;
+    int64_t list_overflows = 0, max_list_length = 0, tree_nodes = 0, tree_max_level = 0, gravity_opened = 0;
 };
 
 /* ------------------------------------------------------------- simulation */
@@ -494,7 +494,7 @@ static inline double soft_acc(double r, double h)
 // Open a cell when edge^2 > theta^2 |x_i - com|^2; leaves are summed
 // particle by particle (i included, which gives the self-potential),
 // accepted cells contribute their monopole.
-static void tree_gravity(const Sim &s, int idx, Particle &pi, int64_t &npp, int64_t &npn)
+static void tree_gravity(const Sim &s, int idx, Particle &pi, int64_t &npp, int64_t &npn, int64_t &nopen)
 {
     const Tree &t = s.tree;
     const Node &nd = s.tree.node(idx);
@@ -503,6 +503,7 @@ static void tree_gravity(const Sim &s, int idx, Particle &pi, int64_t &npp, int6
     const double d2 = norm2(d);
     const double G = t.G;
     if(l2 > t.theta2 * d2) {
+        ++nopen;
         if(nd.leaf) {
             for(int pj = nd.first; pj >= 0; pj = s.parts[pj].next) {
                 const Particle &q = s.parts[pj];
@@ -514,7 +515,7 @@ static void tree_gravity(const Sim &s, int idx, Particle &pi, int64_t &npp, int6
             }
         } else {
             for(int c = 0; c < NCHILD; ++c)
-                if(nd.child[c] >= 0) tree_gravity(s, nd.child[c], pi, npp, npn);
+                if(nd.child[c] >= 0) tree_gravity(s, nd.child[c], pi, npp, npn, nopen);
         }
     } else {
         const double rinv = 1.0 / std::sqrt(d2);
@@ -958,8 +959,8 @@ static int gravity_force(Sim &s)
     if(!s.p.gravity_is_valid) return SPHGPU_OK;
     const int n = s.n;
     const double G = s.p.gravity_constant;
-    int64_t npp = 0, npn = 0;
-#pragma omp parallel for reduction(+ : npp, npn)
+    int64_t npp = 0, npn = 0, nopen = 0;
+#pragma omp parallel for reduction(+ : npp, npn, nopen)
     for(int i = 0; i < n; ++i) {
         Particle &pi = s.parts[i];
         if(s.exhaustive) {
@@ -978,11 +979,12 @@ static int gravity_force(Sim &s)
         } else {
             // BHTree::tree_force (src/bhtree.cpp:128-132)
             pi.phi = 0.0;
-            tree_gravity(s, -1, pi, npp, npn);
+            tree_gravity(s, -1, pi, npp, npn, nopen);
         }
     }
     s.stats.gravity_pp += npp;
     s.stats.gravity_pn += npn;
+    s.stats.gravity_opened += nopen;
     return SPHGPU_OK;
 }
 
@@ -1332,6 +1334,7 @@ int sphref_get_stats(void *ctx, sphgpu_stats *st)
     st->search_truncations = s.stats.search_truncations; st->list_overflows = s.stats.list_overflows;
     st->max_list_length = s.stats.max_list_length; st->tree_nodes = s.stats.tree_nodes;
     st->tree_max_level = s.stats.tree_max_level;
+    st->gravity_opened = s.stats.gravity_opened;
     return SPHGPU_OK;
 }
 

@@ -560,6 +560,7 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->force_pairs = k[C_FORCE_PAIRS];
     st->gravity_pp = k[C_GRAV_PP];
     st->gravity_pn = k[C_GRAV_PN];
+    st->gravity_opened = k[C_GRAV_OPEN];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];

@@ -44,7 +44,7 @@ struct TreeDev {
 // Device-side counters (int64), reduced per wave before the atomic.
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
-    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, NCOUNTERS
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, NCOUNTERS
 };
 
 // Device scalars.

@@ -28,7 +28,7 @@ template <int D> struct GravityVisitor {
     double phi = 0.0;
     double acc[D];
     double d[D], d2 = 0.0;
-    int64_t npp = 0, npn = 0;
+    int64_t npp = 0, npn = 0, nopen = 0;
 
     __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_)
     {
@@ -48,7 +48,9 @@ template <int D> struct GravityVisitor {
         d2 = d[0] * d[0];
 #pragma unroll
         for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
-        return l2 > theta2 * d2;
+        const bool o = l2 > theta2 * d2;
+        nopen += o ? 1 : 0;
+        return o;
     }
 
     __device__ __forceinline__ void far(int node)
@@ -107,10 +109,11 @@ template <int D> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s
 #pragma unroll
         for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k];
     }
-    const int64_t pp = wave_sum(v.npp), pn = wave_sum(v.npn);
+    const int64_t pp = wave_sum(v.npp), pn = wave_sum(v.npn), po = wave_sum(v.nopen);
     if(wave_lane() == 0) {
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)pp);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)pn);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)po);
     }
 }
 

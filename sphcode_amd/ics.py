@@ -177,7 +177,7 @@ def rollout(n_side: int = 64, seed: int = 1, precision: int = _abi.FP64):
 
 
 def sedov(n_side: int = 256, seed: int = 1, n_hot: int = 64, energy: float = 1.0,
-          precision: int = _abi.FP64):
+          precision: int = _abi.FP64, u_background: float = 1e-5):
     """3D Sedov-type blast in the periodic unit cube, GSPH second order.
 
     n_side^3 lattice with Gaussian jitter of 0.05 spacings, rho = 1,
@@ -196,7 +196,7 @@ def sedov(n_side: int = 256, seed: int = 1, n_hot: int = 64, energy: float = 1.0
     n = pos.shape[0]
     mass = np.full(n, 1.0 / n)
     dens = np.ones(n)
-    ene = np.full(n, 1e-5)
+    ene = np.full(n, u_background)
     d = pos - 0.5
     r = np.sqrt((d * d).sum(1))
     hot = np.argpartition(r, n_hot)[:n_hot]
