@@ -154,6 +154,8 @@ typedef struct sphgpu_stats {
     int64_t tree_nodes;           /* nodes of the last tree                   */
     int64_t tree_max_level;       /* deepest level of the last tree           */
     int64_t gravity_opened;       /* cells opened by the gravity walk         */
+    int64_t search_nodes;         /* cells visited by neighbour-search walks  */
+    int64_t search_candidates;    /* particles tested by neighbour searches   */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

@@ -1370,6 +1370,46 @@ int sphref_neighbor_sets(void *ctx, int kind, int32_t *counts, int32_t *lists, i
     return SPHGPU_OK;
 }
 
+// ---- building blocks, exported for the oracle's own tests ----------------
+// W(r, h), grad W = r_ij * c (evaluated with r_ij = (r, 0, ...)), dW/dh.
+int sphref_kernel_eval(int kernel, double r, double h, double *w, double *c, double *dhw)
+{
+    Kernel k{kernel};
+    Vec rij;
+    rij[0] = r;
+    *w = k.w(r, h);
+    const Vec g = k.dw(rij, r, h);
+    *c = r != 0.0 ? g[0] / r : 0.0;
+    *dhw = k.dhw(r, h);
+    return SPHGPU_OK;
+}
+
+// Hernquist-Katz softened potential kernel f and force kernel g.
+int sphref_softening(double r, double h, double *f, double *g)
+{
+    *f = soft_pot(r, h);
+    *g = soft_acc(r, h);
+    return SPHGPU_OK;
+}
+
+int sphref_hll(const double *L, const double *R, double *pstar, double *vstar)
+{
+    hll(L, R, *pstar, *vstar);
+    return SPHGPU_OK;
+}
+
+double sphref_van_leer(double a, double b) { return van_leer(a, b); }
+
+int sphref_min_image(void *ctx, const double *a, const double *b, double *out)
+{
+    Sim &s = *static_cast<Sim *>(ctx);
+    Vec va, vb;
+    for(int k = 0; k < D; ++k) { va[k] = a[k]; vb[k] = b[k]; }
+    const Vec d = s.box.rij(va, vb);
+    for(int k = 0; k < D; ++k) out[k] = d[k];
+    return SPHGPU_OK;
+}
+
 // Tree nodes in depth-first pre-order (children in slot order).
 int sphref_tree_info(void *ctx, int64_t *num_nodes, int32_t *level, int32_t *count, double *edge,
                      double *center, double *mass, double *com, int64_t cap)

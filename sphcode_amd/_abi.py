@@ -97,7 +97,8 @@ class Stats(C.Structure):
     _fields_ = [(name, C.c_int64) for name in (
         "steps", "gather_pairs", "force_pairs", "gravity_pp", "gravity_pn",
         "newton_failures", "newton_iterations", "search_truncations",
-        "list_overflows", "max_list_length", "tree_nodes", "tree_max_level", "gravity_opened")]
+        "list_overflows", "max_list_length", "tree_nodes", "tree_max_level", "gravity_opened",
+        "search_nodes", "search_candidates")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

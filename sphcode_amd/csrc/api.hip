@@ -72,7 +72,7 @@ static void free_buffers(Ctx &c)
     fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
-    fr(c.parent); fr(c.nchild); fr(c.arrive); fr(c.msum);
+    fr(c.parent); fr(c.nchild); fr(c.arrive); fr(c.msum); fr(c.groups); fr(c.num_groups);
     fr(c.list); fr(c.list_r); fr(c.list_cnt);
     fr(c.rec_gather); fr(c.rec_gather_cnt); fr(c.rec_force); fr(c.rec_force_cnt);
     fr(c.partial);
@@ -561,6 +561,8 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->gravity_pp = k[C_GRAV_PP];
     st->gravity_pn = k[C_GRAV_PN];
     st->gravity_opened = k[C_GRAV_OPEN];
+    st->search_nodes = k[C_SEARCH_NODES];
+    st->search_candidates = k[C_SEARCH_CAND];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];

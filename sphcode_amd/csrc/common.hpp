@@ -235,6 +235,22 @@ template <class T> __device__ __forceinline__ T soft_pot(T r, T einv)
     return T(1) / r;
 }
 
+// Both softened kernels at once, without branches, given u = r/e, 1/e and
+// 1/r (1/r may be 0 at r = 0, where only the inner piece is used).
+template <class T> __device__ __forceinline__ void soft_both(T u, T einv, T rinv, T &pot, T &acc)
+{
+    const T u2 = u * u;
+    const T u3 = u2 * u;
+    const T e3 = einv * einv * einv;
+    const T r3 = rinv * rinv * rinv;
+    const T pot_in = (T(-0.5) * u2 * (T(1.0 / 3.0) - T(3.0 / 20.0) * u2 + u3 * T(1.0 / 20.0)) + T(1.4)) * einv;
+    const T acc_in = (T(4.0 / 3.0) - T(1.2) * u2 + T(0.5) * u3) * e3;
+    const T pot_mid = T(-1.0 / 15.0) * rinv + (-u2 * (T(4.0 / 3.0) - u + T(0.3) * u2 - u3 * T(1.0 / 30.0)) + T(1.6)) * einv;
+    const T acc_mid = (T(-1.0 / 15.0) + T(8.0 / 3.0) * u3 - T(3) * u3 * u + T(1.2) * u3 * u2 - u3 * u3 * T(1.0 / 6.0)) * r3;
+    pot = u < T(1) ? pot_in : (u < T(2) ? pot_mid : rinv);
+    acc = u < T(1) ? acc_in : (u < T(2) ? acc_mid : r3);
+}
+
 template <class T> __device__ __forceinline__ T soft_acc(T r, T einv)
 {
     const T u = r * einv;

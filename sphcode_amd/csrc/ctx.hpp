@@ -44,7 +44,7 @@ struct TreeDev {
 // Device-side counters (int64), reduced per wave before the atomic.
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
-    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, NCOUNTERS
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, NCOUNTERS
 };
 
 // Device scalars.
@@ -78,6 +78,11 @@ struct Ctx {
     TreeDev t{};
     int *parent = nullptr, *nchild = nullptr, *arrive = nullptr;
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
+    // search groups: particle ranges of at most 64 particles, each inside
+    // the deepest cell holding at most 64 particles
+    int2 *groups = nullptr;
+    int *num_groups = nullptr;   // device scalar
+    int64_t group_cap = 0;
 
     // neighbour lists: list[(wave * cap + k) * 64 + lane]
     int *list = nullptr;

@@ -56,7 +56,7 @@ template <int D> struct GravityVisitor {
     __device__ __forceinline__ void far(int node)
     {
         const double m = t.mom[node].w;
-        const double rinv = 1.0 / sqrt(d2);
+        const double rinv = rsqrt(d2);
         const double gm = G * m;
         phi -= gm * rinv;
         const double c = gm * rinv * rinv * rinv;
@@ -75,10 +75,22 @@ template <int D> struct GravityVisitor {
             const double einv_j = 2.0 * s.f[F_HINV][j];
             if(open) {
                 double rij[D];
-                const double rr = sqrt(sep2<D>(box, xi, xj, rij));
-                const double gm = G * mj;
-                phi -= gm * (soft_pot(rr, einv_i) + soft_pot(rr, einv_j)) * 0.5;
-                const double c = gm * (soft_acc(rr, einv_i) + soft_acc(rr, einv_j)) * 0.5;
+                const double r2 = sep2<D>(box, xi, xj, rij);
+                const double rinv = r2 > 0.0 ? rsqrt(r2) : 0.0;
+                const double rr = r2 * rinv;
+                // far field of both softening lengths unless one is within 2e
+                double pot = rinv + rinv, acc3 = 2.0 * rinv * rinv * rinv;
+                const double ui = rr * einv_i, uj = rr * einv_j;
+                if(ui < 2.0 || uj < 2.0) {
+                    double pi_, ai, pj, aj;
+                    soft_both(ui, einv_i, rinv, pi_, ai);
+                    soft_both(uj, einv_j, rinv, pj, aj);
+                    pot = pi_ + pj;
+                    acc3 = ai + aj;
+                }
+                const double gm = G * mj * 0.5;
+                phi -= gm * pot;
+                const double c = gm * acc3;
 #pragma unroll
                 for(int k = 0; k < D; ++k) acc[k] -= rij[k] * c;
                 ++npp;

@@ -1,28 +1,34 @@
-// Neighbour search: one wave-coherent tree walk per 64 Morton-consecutive
-// particles, writing per-particle candidate lists (BHTree::neighbor_search,
-// src/bhtree.cpp:114-126, 234-270).
+// Neighbour search (BHTree::neighbor_search, src/bhtree.cpp:114-126,
+// 234-270) for 64 Morton-consecutive particles per wave, in two phases:
 //
-// Gather search (PreInteraction): H = h_guess * kernel_ratio with
-// h_guess = (N_ngb m / (rho A_D))^(1/D) (src/pre_interaction.cpp:61-64);
-// a cell is pruned when max_k |x_i - c|_k^2 > (edge/2 + H)^2, and a leaf
-// member j is a candidate when |x_i - x_j|^2 < H^2.  The candidate's |r_ij|
-// is stored beside its index for the Newton iterations.
-// Symmetric search (FluidForce): H = max(h_i, largest h in the cell)
-// (src/bhtree.cpp:237).
+//  1. Group walk.  The wave walks the pre-order tree once with a single,
+//     wave-uniform cursor, keeping every leaf whose cube can hold a
+//     candidate of any of its particles: per axis, the minimum-image
+//     distance between the group's box centre and the cell centre must not
+//     exceed group half-extent + edge/2 + H, with H the group's largest
+//     search radius (symmetric search: max(that, cell's largest h)).  This
+//     is implied by the reference's per-particle test, so no candidate of
+//     any lane is lost.  Leaves are collected into LDS.
+//  2. Candidate tiles.  The collected leaves' particles (contiguous ranges
+//     in key order) are staged 64 at a time into LDS with coalesced loads;
+//     every lane then applies the reference's leaf test to each staged
+//     particle: |x_i - x_j|^2 < H_i^2 with H_i = h_guess * kernel_ratio
+//     (gather, src/pre_interaction.cpp:61-70) or max(h_i, leaf's largest h)
+//     (symmetric, src/bhtree.cpp:237, 256).
 //
-// The reference sorts each list by distance (src/bhtree.cpp:120-124) and
-// stops at the first r >= h; the pair loops here test r < h on the
-// unsorted list instead, which selects the same set.
+// The sets are the reference's; the reference then sorts each list by
+// distance (src/bhtree.cpp:120-124) and stops at the first r >= h, whereas
+// the pair loops here test r < h on the unsorted list -- the same set.
 //
 // List layout: entry k of the particle in slot (wave w, lane l) lives at
 // (w * cap + k) * 64 + l, so a wave's k-th loads are one contiguous line.
-#include "walk.hpp"
+#include "ctx.hpp"
 
 namespace sphgpu {
 
 struct SearchArgs {
     int *list;
-    double *list_r;      // may be null
+    double *list_r;      // |r_ij| of each candidate (gather search only)
     int *cnt;
     int cap;
     int guess;           // 1: set sml = h_guess * ratio first
@@ -31,116 +37,235 @@ struct SearchArgs {
     int64_t *counters;
 };
 
-template <int D, bool IJ> struct SearchVisitor {
-    const TreeDev &t;
-    const BoxDev &box;
-    const State &s;
-    double xi[D];
-    double h, H, H2;
-    int count = 0;
-    int cap;
-    int *out;
-    double *outr;
+constexpr int SW = BLOCK / WAVE;  // waves per block
+constexpr int LEAFCAP = 128;      // leaves collected per flush
 
-    __device__ __forceinline__ SearchVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_) {}
-
-    __device__ __forceinline__ bool open(int node)
-    {
-#pragma clang fp contract(off)
-        const double4 g = t.geo[node];
-        if constexpr(IJ) {
-            const double k = t.ks[node];
-            H = h < k ? k : h;
-        } else {
-            H = h;
-        }
-        H2 = H * H;
-        const double l2 = sq(g.w * 0.5 + H);
-        double d[D];
-        d[0] = xi[0] - g.x;
-        if constexpr(D > 1) d[1] = xi[1] - g.y;
-        if constexpr(D > 2) d[2] = xi[2] - g.z;
-        min_image<D>(box, d);
-        double dmax = d[0] * d[0];
+// Wave-wide exclusive prefix sum.
+__device__ __forceinline__ int wave_exclusive_scan(int v, int &total)
+{
+    const int lane = wave_lane();
+    int x = v;
 #pragma unroll
-        for(int k = 1; k < D; ++k) {
-            const double dk = d[k] * d[k];
-            dmax = dk > dmax ? dk : dmax;
-        }
-        return dmax <= l2;
+    for(int off = 1; off < WAVE; off <<= 1) {
+        const int y = __shfl_up(x, off, WAVE);
+        if(lane >= off) x += y;
     }
+    total = __shfl(x, WAVE - 1, WAVE);
+    return x - v;
+}
 
-    __device__ __forceinline__ void far(int) {}
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
-    __device__ __forceinline__ void leaf(int4 r, bool open)
-    {
-        for(int j = r.x; j < r.y; ++j) {
-            double xj[D], d[D];
-#pragma unroll
-            for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-            const double r2 = sep2<D>(box, xi, xj, d);
-            if(open && r2 < H2) {
-                if(count < cap) {
-                    out[(size_t)count * WAVE] = j;
-                    if(outr) outr[(size_t)count * WAVE] = sqrt(r2);
-                }
-                ++count;
-            }
-        }
-    }
-};
+// Minimum-image separation of one axis (include/periodic.hpp:34-59).
+__device__ __forceinline__ double min_image1(const BoxDev &b, int k, double d)
+{
+    if(!b.on) return d;
+    const double d2 = d + b.len[k], d3 = d - b.len[k];
+    const double a1 = fabs(d), a2 = fabs(d2), a3 = fabs(d3);
+    double r = d3;
+    if(a2 <= a3 && a2 <= a1) r = d2;
+    if(a1 <= a2 && a1 <= a3) r = d;
+    return r;
+}
 
 template <int D, bool IJ>
-__global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDev box, int n, SearchArgs a)
+__global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDev box, const int2 *groups, const int *num_groups,
+                                                      SearchArgs a)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const bool valid = i < n;
-    const size_t wave = (size_t)i / WAVE;
-    const int lane = i % WAVE;
-    SearchVisitor<D, IJ> v(t, box, s);
-    v.cap = a.cap;
-    v.out = a.list + wave * a.cap * WAVE + lane;
-    v.outr = a.list_r ? a.list_r + wave * a.cap * WAVE + lane : nullptr;
-    v.h = 0.0;
-    if(valid) {
+    __shared__ int l_begin[SW][LEAFCAP];
+    __shared__ int l_pre[SW][LEAFCAP + 1];
+    __shared__ double l_ks[SW][LEAFCAP];
+    __shared__ double c_x[SW][3][WAVE];
+    __shared__ double c_ks[SW][WAVE];
+    __shared__ int c_j[SW][WAVE];
+
+    const int w = uniform(threadIdx.x / WAVE);
+    const int lane = wave_lane();
+    const int ngroups = uniform(*num_groups);
+    const int nwaves = gridDim.x * SW;
+    for(int grp = blockIdx.x * SW + w; grp < ngroups; grp += nwaves) {
+    const int2 gr = make_int2(uniform(groups[grp].x), uniform(groups[grp].y));
+    const int i = gr.x + lane;
+    const bool valid = i < gr.y;
+
+    double xi[D], h = 0.0;
 #pragma unroll
-        for(int k = 0; k < D; ++k) v.xi[k] = s.f[F_X + k][i];
+    for(int k = 0; k < D; ++k) xi[k] = valid ? s.f[F_X + k][i] : 0.0;
+    if(valid) {
         if(a.guess) {
             // p_i.sml = (N_ngb m / (rho A))^(1/D) * ratio (src/pre_interaction.cpp:61-64)
             const double m = s.f[F_MASS][i];
             const double rho = s.f[F_DENS][i];
-            v.h = root_dim<D>(a.nngb * m / (rho * UnitVolume<D>::value)) * a.ratio;
-            s.f[F_SML][i] = v.h;
+            h = root_dim<D>(a.nngb * m / (rho * UnitVolume<D>::value)) * a.ratio;
+            s.f[F_SML][i] = h;
         } else {
-            v.h = s.f[F_SML][i];
+            h = s.f[F_SML][i];
         }
     }
-    tree_walk(t, *t.num_nodes, valid, v);
+    const double h2_own = h * h;
+
+    // group box: centre, half extent, largest radius (wave-uniform)
+    double gc[D], ge[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) {
+        const double lo = wave_minv(valid ? xi[k] : 1.7976931348623157e308);
+        const double hi = wave_maxv(valid ? xi[k] : -1.7976931348623157e308);
+        gc[k] = 0.5 * (lo + hi);
+        ge[k] = 0.5 * (hi - lo) * (1.0 + 1e-12) + 1e-300;
+    }
+    const double hmax = wave_maxv(valid ? h : 0.0);
+
+    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
+    int *out = a.list + base;
+    double *outr = a.list_r ? a.list_r + base : nullptr;
+    int count = 0;
+
+    const int total = uniform(*t.num_nodes);
+    int node = 0;
+    int64_t visited = 0, tested = 0;
+    while(node < total) {
+        // ---- phase 1: collect up to LEAFCAP leaves
+        int nleaf = 0;
+        while(node < total && nleaf < LEAFCAP) {
+            node = uniform(node);
+            const double4 g = t.geo[node];
+            const int4 r = t.rng[node];
+            ++visited;
+            double H = hmax;
+            double ks = 0.0;
+            if constexpr(IJ) {
+                ks = t.ks[node];
+                H = ks > H ? ks : H;
+            }
+            const double reach = g.w * 0.5 + H;
+            const double gcen[3] = {g.x, g.y, g.z};
+            bool overlap = true;
+#pragma unroll
+            for(int k = 0; k < D; ++k) {
+                const double dd = min_image1(box, k, gc[k] - gcen[k]);
+                overlap = overlap && fabs(dd) <= (ge[k] + reach) * (1.0 + 1e-12);
+            }
+            const int skip = r.z > node ? r.z : node + 1;
+            if(overlap && (r.w & 256)) {
+                if(lane == 0) {
+                    l_begin[w][nleaf] = r.x;
+                    l_pre[w][nleaf] = r.y - r.x;
+                    l_ks[w][nleaf] = ks;
+                }
+                ++nleaf;
+                node = skip;
+            } else {
+                node = overlap ? node + 1 : skip;
+            }
+        }
+        wave_lds_sync();
+        // prefix of leaf sizes (each lane owns two slots)
+        int c0 = 0, c1 = 0;
+        if(2 * lane < nleaf) c0 = l_pre[w][2 * lane];
+        if(2 * lane + 1 < nleaf) c1 = l_pre[w][2 * lane + 1];
+        int ncand = 0;
+        const int ex = wave_exclusive_scan(c0 + c1, ncand);
+        ncand = uniform(ncand);
+        wave_lds_sync();
+        if(2 * lane < nleaf) l_pre[w][2 * lane] = ex;
+        if(2 * lane + 1 < nleaf) l_pre[w][2 * lane + 1] = ex + c0;
+        if(lane == 0) l_pre[w][nleaf] = ncand;
+        wave_lds_sync();
+
+        // ---- phase 2: candidate tiles of 64
+        for(int tb = 0; tb < ncand; tb += WAVE) {
+            // stage: each lane fetches one candidate and keeps it only if it
+            // lies within the group's largest radius of the group box (the
+            // box contains every x_i, so each lane's own test implies this)
+            const int f = tb + lane;
+            bool keep = false;
+            int j = 0;
+            double xj[D], ksj = 0.0;
+            if(f < ncand) {
+                int lo = 0, hi = nleaf - 1;  // last leaf with l_pre <= f
+                while(lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if(l_pre[w][mid] <= f) lo = mid;
+                    else hi = mid - 1;
+                }
+                j = l_begin[w][lo] + (f - l_pre[w][lo]);
+                double d2b = 0.0;
+#pragma unroll
+                for(int k = 0; k < D; ++k) {
+                    xj[k] = s.f[F_X + k][j];
+                    const double ex = fabs(min_image1(box, k, xj[k] - gc[k])) - ge[k];
+                    d2b += ex > 0.0 ? ex * ex : 0.0;
+                }
+                double Hc = hmax;
+                if constexpr(IJ) {
+                    ksj = l_ks[w][lo];
+                    Hc = ksj > Hc ? ksj : Hc;
+                }
+                keep = d2b <= Hc * Hc * (1.0 + 1e-9);
+            }
+            const uint64_t kept = __ballot(keep);
+            if(keep) {
+                const int pos = __popcll(kept & ((1ull << lane) - 1ull));
+                c_j[w][pos] = j;
+#pragma unroll
+                for(int k = 0; k < D; ++k) c_x[w][k][pos] = xj[k];
+                if constexpr(IJ) c_ks[w][pos] = ksj;
+            }
+            const int m = uniform(__popcll(kept));
+            tested += m;
+            wave_lds_sync();
+            if(valid) {
+                for(int c = 0; c < m; ++c) {
+                    double xj[D], d[D];
+#pragma unroll
+                    for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
+                    const double r2 = sep2<D>(box, xi, xj, d);
+                    double lim = h2_own;
+                    if constexpr(IJ) {
+                        const double kl = c_ks[w][c];
+                        const double Hc = h < kl ? kl : h;
+                        lim = Hc * Hc;
+                    }
+                    if(r2 < lim) {
+                        if(count < a.cap) {
+                            out[(size_t)count * WAVE] = c_j[w][c];
+                            if(outr) outr[(size_t)count * WAVE] = sqrt(r2);
+                        }
+                        ++count;
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+
     int over = 0, len = 0;
     if(valid) {
-        a.cnt[i] = v.count < a.cap ? v.count : a.cap;
-        over = v.count > a.cap ? 1 : 0;
-        len = v.count;
+        a.cnt[i] = count < a.cap ? count : a.cap;
+        over = count > a.cap ? 1 : 0;
+        len = count;
     }
     over = wave_sum(over);
     len = wave_maxv(len);
-    if(wave_lane() == 0) {
+    if(lane == 0) {
+        atomicAdd((unsigned long long *)&a.counters[C_SEARCH_NODES], (unsigned long long)visited);
+        atomicAdd((unsigned long long *)&a.counters[C_SEARCH_CAND], (unsigned long long)tested);
         if(over) atomicAdd((unsigned long long *)&a.counters[C_OVERFLOW], (unsigned long long)over);
         atomicMax((unsigned long long *)&a.counters[C_MAXLIST], (unsigned long long)len);
     }
+    }  // groups
 }
 
 template <int D, bool IJ> static void launch_search(Ctx &c, const SearchArgs &a)
 {
-    BoxDev box{};
-    box.on = c.p.periodic_is_valid;
-    for(int k = 0; k < 3; ++k) {
-        box.lo[k] = c.p.range_min[k];
-        box.hi[k] = c.p.range_max[k];
-        box.len[k] = c.p.range_max[k] - c.p.range_min[k];
-    }
-    const int n = (int)c.n;
-    hipLaunchKernelGGL((search_kernel<D, IJ>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a);
+    const BoxDev box = make_box(c);
+    const int blocks = std::min(grid_for(c.n), 4096);
+    hipLaunchKernelGGL((search_kernel<D, IJ>), dim3(blocks), dim3(BLOCK), 0, c.stream, c.s, c.t, box, c.groups, c.num_groups, a);
 }
 
 template <bool IJ> static int search_dispatch(Ctx &c, const SearchArgs &a)

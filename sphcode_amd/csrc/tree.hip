@@ -345,6 +345,32 @@ __global__ void __launch_bounds__(BLOCK) moments_kernel(State s, TreeDev t, cons
 }
 
 // Largest smoothing length per subtree (BHNode::set_kernel).
+// Search groups: the deepest cells holding at most GROUP particles (a cell
+// whose parent holds more), cut into ranges of GROUP when a leaf is larger.
+constexpr int GROUP = 64;
+
+__global__ void reset_groups_kernel(int *num_groups) { *num_groups = 0; }
+
+__global__ void __launch_bounds__(BLOCK) groups_kernel(TreeDev t, const int *parent, int2 *groups, int *num_groups, int64_t group_cap)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int total = *t.num_nodes;
+    if(p >= total) return;
+    const int4 r = t.rng[p];
+    const int cnt = r.y - r.x;
+    const int q = parent[p];
+    const int pc = q >= 0 && q < total ? t.rng[q].y - t.rng[q].x : 0x7fffffff;
+    const bool leaf = (r.w & 256) != 0;
+    if(!(pc > GROUP && (cnt <= GROUP || leaf))) return;
+    const int pieces = (cnt + GROUP - 1) / GROUP;
+    const int g0 = atomicAdd(num_groups, pieces);
+    for(int k = 0; k < pieces; ++k) {
+        const int b = r.x + k * GROUP;
+        const int e = b + GROUP < r.y ? b + GROUP : r.y;
+        if(g0 + k < group_cap) groups[g0 + k] = make_int2(b, e);
+    }
+}
+
 __global__ void __launch_bounds__(BLOCK) kernel_size_kernel(State s, TreeDev t, const int *parent, const int *nchild, int *arrive)
 {
     const int p = blockIdx.x * BLOCK + threadIdx.x;
@@ -394,6 +420,9 @@ int tree_alloc(Ctx &c)
     SPHGPU_HIP(hipMalloc(&c.nchild, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.arrive, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.msum, c.node_cap * sizeof(double4)));
+    c.group_cap = c.node_cap + n / 64 + 1;
+    SPHGPU_HIP(hipMalloc(&c.groups, c.group_cap * sizeof(int2)));
+    SPHGPU_HIP(hipMalloc(&c.num_groups, sizeof(int)));
     size_t bytes = 0;
     const unsigned key_bits = (unsigned)(c.p.tree_max_level * c.D);
     SPHGPU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c.keys, c.keys_sorted, c.idx, c.perm, (unsigned)n, 0, key_bits, c.stream));
@@ -444,6 +473,8 @@ template <int D> static int tree_build_d(Ctx &c)
     hipLaunchKernelGGL(reset_parent_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.node_cap, c.parent);
     hipLaunchKernelGGL(link_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.nchild, c.arrive);
     hipLaunchKernelGGL(moments_kernel<D>, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.s, c.t, c.parent, c.nchild, c.arrive, c.msum);
+    hipLaunchKernelGGL(reset_groups_kernel, dim3(1), dim3(1), 0, c.stream, c.num_groups);
+    hipLaunchKernelGGL(groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.groups, c.num_groups, c.group_cap);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
