@@ -68,7 +68,10 @@ static void free_buffers(Ctx &c)
     for(int f = 0; f < NF; ++f) {
         fr(c.s.f[f]);
         fr(c.alt.f[f]);
+        fr(c.s.g[f]);
+        c.alt.g[f] = nullptr;
     }
+    fr(c.list_rf);
     fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
@@ -84,10 +87,12 @@ static int alloc_buffers(Ctx &c, int64_t n)
     free_buffers(c);
     const int64_t np = (n + WAVE - 1) / WAVE * WAVE;
     c.cap_n = np;
+    const bool mixed = c.p.precision == SPHGPU_MIXED;
     for(int f = 0; f < NF; ++f) {
         if(!field_used(c, f)) continue;
         SPHGPU_HIP(hipMalloc(&c.s.f[f], np * sizeof(double)));
         SPHGPU_HIP(hipMalloc(&c.alt.f[f], np * sizeof(double)));
+        if(mixed && f >= F_V) SPHGPU_HIP(hipMalloc(&c.s.g[f], np * sizeof(float)));
     }
     SPHGPU_HIP(hipMalloc(&c.s.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.alt.id, np * sizeof(int)));
@@ -95,7 +100,8 @@ static int alloc_buffers(Ctx &c, int64_t n)
     SPHGPU_HIP(hipMalloc(&c.alt.nb, np * sizeof(int)));
     const size_t lsz = (size_t)np * (size_t)c.cap;
     SPHGPU_HIP(hipMalloc(&c.list, lsz * sizeof(int)));
-    SPHGPU_HIP(hipMalloc(&c.list_r, lsz * sizeof(double)));
+    if(mixed) SPHGPU_HIP(hipMalloc(&c.list_rf, lsz * sizeof(float)));
+    else SPHGPU_HIP(hipMalloc(&c.list_r, lsz * sizeof(double)));
     SPHGPU_HIP(hipMalloc(&c.list_cnt, np * sizeof(int)));
     const int64_t npart = std::max<int64_t>(grid_for(np), 1024) * 8 + 64;
     SPHGPU_HIP(hipMalloc(&c.partial, npart * sizeof(double)));

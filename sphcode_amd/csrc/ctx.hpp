@@ -27,8 +27,35 @@ enum Field : int {
 // Kernel-argument view of the state.
 struct State {
     double *f[NF];
-    int *id;   // caller index of the particle in each sorted slot
-    int *nb;   // neighbour count (SPHParticle::neighbor)
+    float *g[NF];  // FP32 mirrors read by the mixed-precision pair kernels
+                   // (positions excepted; allocated in mixed mode only)
+    int *id;       // caller index of the particle in each sorted slot
+    int *nb;       // neighbour count (SPHParticle::neighbor)
+};
+
+// FP64 master field, or its FP32 mirror, as the pair arithmetic type.
+template <class T> __device__ __forceinline__ T fld(const State &s, int f, int i)
+{
+    if constexpr(sizeof(T) == 8) return s.f[f][i];
+    else return s.g[f][i];
+}
+
+// Per-particle sums: plain in FP64; Kahan-compensated in FP32.
+template <class T> struct Acc {
+    T s = T(0);
+    __device__ __forceinline__ void add(T x) { s += x; }
+    __device__ __forceinline__ double val() const { return (double)s; }
+};
+template <> struct Acc<float> {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double val() const { return (double)s - (double)c; }
 };
 
 // Octree in depth-first pre-order; node 0 is the root.
@@ -86,7 +113,8 @@ struct Ctx {
 
     // neighbour lists: list[(wave * cap + k) * 64 + lane]
     int *list = nullptr;
-    double *list_r = nullptr;  // |r_ij| of gather candidates
+    double *list_r = nullptr;  // |r_ij| of gather candidates (FP64 mode)
+    float *list_rf = nullptr;  // |r_ij| of gather candidates (mixed mode)
     int *list_cnt = nullptr;
     int *rec_gather = nullptr, *rec_gather_cnt = nullptr;  // recorded sets (debug)
     int *rec_force = nullptr, *rec_force_cnt = nullptr;
@@ -146,6 +174,8 @@ int step_timestep(Ctx &c);       // dt into scal[S_DT]
 int step_predict(Ctx &c);
 int step_correct(Ctx &c);
 int step_energy(Ctx &c, double *out);
+// FP32 mirrors of the listed fields (mixed precision only; no-op in FP64)
+int mirror_fields(Ctx &c, const int *fields, int nfields);
 int reduce_min_partials(Ctx &c, int nblocks, int slot);
 
 }  // namespace sphgpu

@@ -8,6 +8,11 @@
 // accepted cell adds its monopole.  phi is reset and acc is accumulated on
 // top of the hydrodynamic acceleration.  The walk is the wave-coherent
 // pre-order walk of walk.hpp, so each lane keeps its own opening decisions.
+//
+// T is the interaction arithmetic.  The opening test always runs in FP64,
+// so both precisions take the same interaction lists; in mixed mode the
+// separations are taken in FP64 relative to particle i, rounded to FP32,
+// and the sums are Kahan-compensated.
 #include "walk.hpp"
 
 namespace sphgpu {
@@ -18,23 +23,19 @@ struct GravArgs {
     int64_t *counters;
 };
 
-template <int D> struct GravityVisitor {
+template <int D, class T> struct GravityVisitor {
     const TreeDev &t;
     const BoxDev &box;
     const State &s;
     double G, theta2;
     double xi[D];
-    double einv_i;   // 2 / h_i
-    double phi = 0.0;
-    double acc[D];
+    T einv_i;   // 2 / h_i
+    Acc<T> phi;
+    Acc<T> acc[D];
     double d[D], d2 = 0.0;
     int64_t npp = 0, npn = 0, nopen = 0;
 
-    __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_)
-    {
-#pragma unroll
-        for(int k = 0; k < D; ++k) acc[k] = 0.0;
-    }
+    __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_) {}
 
     __device__ __forceinline__ bool open(int node)
     {
@@ -55,13 +56,12 @@ template <int D> struct GravityVisitor {
 
     __device__ __forceinline__ void far(int node)
     {
-        const double m = t.mom[node].w;
-        const double rinv = rsqrt(d2);
-        const double gm = G * m;
-        phi -= gm * rinv;
-        const double c = gm * rinv * rinv * rinv;
+        const T gm = (T)(G * t.mom[node].w);
+        const T rinv = rsqrt((T)d2);
+        phi.add(-(gm * rinv));
+        const T c = gm * rinv * rinv * rinv;
 #pragma unroll
-        for(int k = 0; k < D; ++k) acc[k] -= d[k] * c;
+        for(int k = 0; k < D; ++k) acc[k].add(-((T)d[k] * c));
         ++npn;
     }
 
@@ -71,61 +71,79 @@ template <int D> struct GravityVisitor {
             double xj[D];
 #pragma unroll
             for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-            const double mj = s.f[F_MASS][j];
-            const double einv_j = 2.0 * s.f[F_HINV][j];
+            const T mj = fld<T>(s, F_MASS, j);
+            const T einv_j = T(2) * fld<T>(s, F_HINV, j);
             if(open) {
-                double rij[D];
-                const double r2 = sep2<D>(box, xi, xj, rij);
-                const double rinv = r2 > 0.0 ? rsqrt(r2) : 0.0;
-                const double rr = r2 * rinv;
+                double dd[D];
+                const double r2d = sep2<D>(box, xi, xj, dd);
+                T rij[D];
+                T r2 = T(0);
+#pragma unroll
+                for(int k = 0; k < D; ++k) {
+                    rij[k] = (T)dd[k];
+                    r2 += rij[k] * rij[k];
+                }
+                if constexpr(sizeof(T) == 8) r2 = r2d;
+                const T rinv = r2 > T(0) ? rsqrt(r2) : T(0);
+                const T rr = r2 * rinv;
                 // far field of both softening lengths unless one is within 2e
-                double pot = rinv + rinv, acc3 = 2.0 * rinv * rinv * rinv;
-                const double ui = rr * einv_i, uj = rr * einv_j;
-                if(ui < 2.0 || uj < 2.0) {
-                    double pi_, ai, pj, aj;
+                T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
+                const T ui = rr * einv_i, uj = rr * einv_j;
+                if(ui < T(2) || uj < T(2)) {
+                    T pi_, ai, pj, aj;
                     soft_both(ui, einv_i, rinv, pi_, ai);
                     soft_both(uj, einv_j, rinv, pj, aj);
                     pot = pi_ + pj;
                     acc3 = ai + aj;
                 }
-                const double gm = G * mj * 0.5;
-                phi -= gm * pot;
-                const double c = gm * acc3;
+                const T gm = (T)G * mj * T(0.5);
+                phi.add(-(gm * pot));
+                const T c = gm * acc3;
 #pragma unroll
-                for(int k = 0; k < D; ++k) acc[k] -= rij[k] * c;
+                for(int k = 0; k < D; ++k) acc[k].add(-(rij[k] * c));
                 ++npp;
             }
         }
     }
 };
 
-template <int D> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const bool valid = i < n;
-    GravityVisitor<D> v(t, box, s);
+    GravityVisitor<D, T> v(t, box, s);
     v.G = a.G;
     v.theta2 = a.theta2;
-    v.einv_i = 0.0;
+    v.einv_i = T(0);
     if(valid) {
 #pragma unroll
         for(int k = 0; k < D; ++k) v.xi[k] = s.f[F_X + k][i];
-        v.einv_i = 2.0 * s.f[F_HINV][i];
+        v.einv_i = (T)(2.0 * s.f[F_HINV][i]);
     } else {
 #pragma unroll
         for(int k = 0; k < D; ++k) v.xi[k] = 0.0;
     }
     tree_walk(t, *t.num_nodes, valid, v);
     if(valid) {
-        s.f[F_PHI][i] = v.phi;
+        s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
-        for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k];
+        for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k].val();
     }
     const int64_t pp = wave_sum(v.npp), pn = wave_sum(v.npn), po = wave_sum(v.nopen);
     if(wave_lane() == 0) {
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)pp);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)pn);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)po);
+    }
+}
+
+template <class T> static void gravity_t(Ctx &c, const GravArgs &a, const BoxDev &box)
+{
+    const int n = (int)c.n;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL((gravity_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
     }
 }
 
@@ -137,12 +155,8 @@ int gravity_compute(Ctx &c)
     a.theta2 = c.p.gravity_theta * c.p.gravity_theta;
     a.counters = c.counters;
     const BoxDev box = make_box(c);
-    const int n = (int)c.n;
-    switch(c.D) {
-    case 1: hipLaunchKernelGGL(gravity_kernel<1>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
-    case 2: hipLaunchKernelGGL(gravity_kernel<2>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
-    default: hipLaunchKernelGGL(gravity_kernel<3>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
-    }
+    if(c.p.precision == SPHGPU_MIXED) gravity_t<float>(c, a, box);
+    else gravity_t<double>(c, a, box);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }

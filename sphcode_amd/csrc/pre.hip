@@ -16,6 +16,11 @@
 // the signal velocity, and the velocity derivatives that the reference
 // gathers in a second loop over the same set (Balsara switch and
 // time-dependent viscosity, or the GSPH MUSCL gradients).
+//
+// T is the pair arithmetic: double, or float for the mixed mode, where
+// pair terms are evaluated in FP32 relative to particle i (separations
+// taken in FP64, then rounded), summed with Kahan compensation, and the
+// Newton update, its convergence test and every stored result stay FP64.
 #include "ctx.hpp"
 
 namespace sphgpu {
@@ -23,6 +28,7 @@ namespace sphgpu {
 struct PreArgs {
     const int *list;
     const double *list_r;
+    const float *list_rf;
     const int *cnt;
     int cap;
     double gamma;
@@ -38,49 +44,63 @@ struct PreArgs {
     double *partial;      // per-block min of h / v_sig
 };
 
-template <int D> struct CurlAcc {
-    double c[3] = {0.0, 0.0, 0.0};
-    __device__ __forceinline__ void add(const double a[D], const double b[D], double m)
+template <class T> __device__ __forceinline__ T list_r(const PreArgs &a, size_t k)
+{
+    if constexpr(sizeof(T) == 8) return a.list_r[k];
+    else return a.list_rf[k];
+}
+
+// Writes an FP64 result and, in mixed mode, its FP32 mirror.
+template <class T> __device__ __forceinline__ void put(const State &s, int f, int i, double v)
+{
+    s.f[f][i] = v;
+    if constexpr(sizeof(T) == 4) s.g[f][i] = (float)v;
+}
+
+template <int D, class T> struct CurlAcc {
+    Acc<T> c[3];
+    __device__ __forceinline__ void add(const T a[D], const T b[D], T m)
     {
         if constexpr(D == 3) {
-            c[0] += (a[1] * b[2] - a[2] * b[1]) * m;
-            c[1] += (a[2] * b[0] - a[0] * b[2]) * m;
-            c[2] += (a[0] * b[1] - a[1] * b[0]) * m;
+            c[0].add((a[1] * b[2] - a[2] * b[1]) * m);
+            c[1].add((a[2] * b[0] - a[0] * b[2]) * m);
+            c[2].add((a[0] * b[1] - a[1] * b[0]) * m);
         } else if constexpr(D == 2) {
-            c[2] += (a[0] * b[1] - a[1] * b[0]) * m;
+            c[2].add((a[0] * b[1] - a[1] * b[0]) * m);
         }
     }
     __device__ __forceinline__ double magnitude() const
     {
-        if constexpr(D == 2) return fabs(c[2]);
-        return sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        if constexpr(D == 2) return fabs(c[2].val());
+        const double x = c[0].val(), y = c[1].val(), z = c[2].val();
+        return sqrt(x * x + y * y + z * z);
     }
 };
 
 // Initial smoothing (src/pre_interaction.cpp:171-215): density with
 // h = h_guess over candidates with r < h.
-template <int D, int K>
+template <int D, int K, class T>
 __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, int n, PreArgs a)
 {
-    using Kern = typename KernelOf<D, K, double>::type;
+    using Kern = typename KernelOf<D, K, T>::type;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
-    const double h = s.f[F_SML][i];
+    const T h = (T)s.f[F_SML][i];
     const Kern ker(h);
-    double rho = 0.0;
+    Acc<T> rho;
     for(int k = 0; k < cnt; ++k) {
-        const double r = a.list_r[base + (size_t)k * WAVE];
-        if(r < h) rho += s.f[F_MASS][a.list[base + (size_t)k * WAVE]] * ker.w(ker.q_of(r));
+        const T r = list_r<T>(a, base + (size_t)k * WAVE);
+        if(r < h) rho.add(fld<T>(s, F_MASS, a.list[base + (size_t)k * WAVE]) * ker.w(ker.q_of(r)));
     }
-    s.f[F_DENS][i] = rho;
+    s.f[F_DENS][i] = rho.val();
 }
 
-template <int D, int K, int TYPE>
+template <int D, int K, int TYPE, class T>
 __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, PreArgs a)
 {
-    using Kern = typename KernelOf<D, K, double>::type;
+    using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool NUMBER = TYPE == SPHGPU_DISPH;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const bool valid = i < n;
@@ -100,24 +120,25 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
             for(int it = 0; it < 10 && !done; ++it) {
                 ++iters;
                 const double hb = hn;
-                const Kern ker(hn);
-                double rho = 0.0, drho = 0.0;
+                const T ht = (T)hn;
+                const Kern ker(ht);
+                Acc<T> rho, drho;
                 for(int k = 0; k < cnt; ++k) {
-                    const double r = a.list_r[base + (size_t)k * WAVE];
-                    if(r < hn) {
-                        const double q = ker.q_of(r);
+                    const T r = list_r<T>(a, base + (size_t)k * WAVE);
+                    if(r < ht) {
+                        const T q = ker.q_of(r);
                         if constexpr(NUMBER) {
-                            rho += ker.w(q);
-                            drho += ker.dhw(q);
+                            rho.add(ker.w(q));
+                            drho.add(ker.dhw(q));
                         } else {
-                            const double mj = s.f[F_MASS][a.list[base + (size_t)k * WAVE]];
-                            rho += mj * ker.w(q);
-                            drho += mj * ker.dhw(q);
+                            const T mj = fld<T>(s, F_MASS, a.list[base + (size_t)k * WAVE]);
+                            rho.add(mj * ker.w(q));
+                            drho.add(mj * ker.dhw(q));
                         }
                     }
                 }
-                const double f = rho * pow_dim<D>(hn) - b;
-                const double df = drho * pow_dim<D>(hn) + D * rho * pow_dim1<D>(hn);
+                const double f = rho.val() * pow_dim<D>(hn) - b;
+                const double df = drho.val() * pow_dim<D>(hn) + D * rho.val() * pow_dim1<D>(hn);
                 hn -= f / df;
                 if(fabs(hn - hb) < (hn + hb) * 1e-4) done = true;
             }
@@ -131,147 +152,148 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         }
 
         // final sums over r < h
-        double xi[D], vi[D];
+        double xi[D];
+        T vi[D];
 #pragma unroll
-        for(int k = 0; k < D; ++k) { xi[k] = s.f[F_X + k][i]; vi[k] = s.f[F_V + k][i]; }
+        for(int k = 0; k < D; ++k) { xi[k] = s.f[F_X + k][i]; vi[k] = (T)s.f[F_V + k][i]; }
         const double ui = s.f[F_ENE][i];
         const double ci = s.f[F_SOUND][i];
-        const Kern ker(h);
-        double rho = 0.0, drho = 0.0, nd = 0.0, dnd = 0.0, pb = 0.0, dpb = 0.0;
-        double vsig_max = ci * 2.0;
-        double divv = 0.0;
-        CurlAcc<D> rot;
-        double gd[D], gu[D], gv[D][D];
-#pragma unroll
-        for(int k = 0; k < D; ++k) {
-            gd[k] = 0.0;
-            gu[k] = 0.0;
-#pragma unroll
-            for(int l = 0; l < D; ++l) gv[k][l] = 0.0;
-        }
+        const T ht = (T)h;
+        const Kern ker(ht);
+        Acc<T> rho, drho, nd, dnd, pb, dpb, divv;
+        T vsig_max = (T)(ci * 2.0);
+        CurlAcc<D, T> rot;
+        Acc<T> gd[D], gu[D], gv[D][D];
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
+        const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
         for(int k = 0; k < cnt; ++k) {
-            const double r = a.list_r[base + (size_t)k * WAVE];
-            if(!(r < h)) continue;
+            const T r = list_r<T>(a, base + (size_t)k * WAVE);
+            if(!(r < ht)) continue;
             const int j = a.list[base + (size_t)k * WAVE];
             ++count;
-            const double mj = s.f[F_MASS][j];
-            const double q = ker.q_of(r);
-            const double w = ker.w(q);
+            const T mj = fld<T>(s, F_MASS, j);
+            const T q = ker.q_of(r);
+            const T w = ker.w(q);
             if constexpr(TYPE == SPHGPU_DISPH) {
-                const double dh = ker.dhw(q);
-                const double uj = s.f[F_ENE][j];
-                rho += mj * w;
-                nd += w;
-                pb += mj * uj * w;
-                dpb += mj * uj * dh;
-                dnd += dh;
+                const T dh = ker.dhw(q);
+                const T uj = fld<T>(s, F_ENE, j);
+                rho.add(mj * w);
+                nd.add(w);
+                pb.add(mj * uj * w);
+                dpb.add(mj * uj * dh);
+                dnd.add(dh);
             } else if constexpr(TYPE == SPHGPU_SSPH) {
-                rho += mj * w;
-                drho += mj * ker.dhw(q);
+                rho.add(mj * w);
+                drho.add(mj * ker.dhw(q));
             } else {
-                rho += mj * w;
+                rho.add(mj * w);
             }
-            double rij[D], vij[D];
+            T rij[D], vij[D];
             if(j != i || need_dv) {
-                double xj[D];
+                double xj[D], d[D];
 #pragma unroll
                 for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
-                sep2<D>(box, xi, xj, rij);
+                sep2<D>(box, xi, xj, d);
 #pragma unroll
-                for(int l = 0; l < D; ++l) vij[l] = vi[l] - s.f[F_V + l][j];
+                for(int l = 0; l < D; ++l) {
+                    rij[l] = (T)d[l];
+                    vij[l] = vi[l] - fld<T>(s, F_V + l, j);
+                }
             }
             if(j != i) {
-                double rv = rij[0] * vij[0];
+                T rv = rij[0] * vij[0];
 #pragma unroll
                 for(int l = 1; l < D; ++l) rv += rij[l] * vij[l];
-                const double vs = ci + s.f[F_SOUND][j] - 3.0 * rv / r;
+                const T vs = ci_t + fld<T>(s, F_SOUND, j) - T(3) * rv / r;
                 vsig_max = vs > vsig_max ? vs : vsig_max;
             }
             if(need_dv) {
-                const double cw = ker.dw_coef(q, r);
-                double dw[D];
+                const T cw = ker.dw_coef(q, r);
+                T dw[D];
 #pragma unroll
                 for(int l = 0; l < D; ++l) dw[l] = rij[l] * cw;
                 if constexpr(TYPE == SPHGPU_GSPH) {
-                    const double du = mj * (s.f[F_ENE][j] - ui);
+                    const T du = mj * (fld<T>(s, F_ENE, j) - ui_t);
 #pragma unroll
                     for(int l = 0; l < D; ++l) {
-                        gd[l] += dw[l] * mj;
-                        gu[l] += dw[l] * du;
+                        gd[l].add(dw[l] * mj);
+                        gu[l].add(dw[l] * du);
                     }
 #pragma unroll
                     for(int c = 0; c < D; ++c) {
-                        const double dvc = mj * (-vij[c]);
+                        const T dvc = mj * (-vij[c]);
 #pragma unroll
-                        for(int l = 0; l < D; ++l) gv[c][l] += dw[l] * dvc;
+                        for(int l = 0; l < D; ++l) gv[c][l].add(dw[l] * dvc);
                     }
                 } else {
-                    const double wgt = TYPE == SPHGPU_DISPH ? mj * s.f[F_ENE][j] : mj;
-                    double vd = vij[0] * dw[0];
+                    const T wgt = TYPE == SPHGPU_DISPH ? mj * fld<T>(s, F_ENE, j) : mj;
+                    T vd = vij[0] * dw[0];
 #pragma unroll
                     for(int l = 1; l < D; ++l) vd += vij[l] * dw[l];
-                    divv -= wgt * vd;
+                    divv.add(-(wgt * vd));
                     rot.add(vij, dw, wgt);
                 }
             }
         }
         pairs = count;
         const double gm1 = a.gamma - 1.0;
-        s.f[F_SML][i] = h;
-        s.f[F_HINV][i] = 1.0 / h;
-        s.f[F_DENS][i] = rho;
+        const double rho_d = rho.val();
+        put<T>(s, F_SML, i, h);
+        put<T>(s, F_HINV, i, 1.0 / h);
+        put<T>(s, F_DENS, i, rho_d);
         double pres;
         if constexpr(TYPE == SPHGPU_DISPH) {
-            pres = gm1 * pb;
-            s.f[F_GRADH][i] = h / (D * nd) * dpb / (1.0 + h / (D * nd) * dnd);
+            pres = gm1 * pb.val();
+            const double nv = nd.val();
+            put<T>(s, F_GRADH, i, h / (D * nv) * dpb.val() / (1.0 + h / (D * nv) * dnd.val()));
         } else {
-            pres = gm1 * rho * ui;
-            if constexpr(TYPE == SPHGPU_SSPH) s.f[F_GRADH][i] = 1.0 / (1.0 + h / (D * rho) * drho);
+            pres = gm1 * rho_d * ui;
+            if constexpr(TYPE == SPHGPU_SSPH) put<T>(s, F_GRADH, i, 1.0 / (1.0 + h / (D * rho_d) * drho.val()));
         }
-        s.f[F_PRES][i] = pres;
+        put<T>(s, F_PRES, i, pres);
         s.nb[i] = count;
-        hvs = h / vsig_max;
+        hvs = h / (double)vsig_max;
 
         if constexpr(TYPE == SPHGPU_GSPH) {
             if(a.second_order) {
-                const double rinv = 1.0 / rho;
+                const double rinv = 1.0 / rho_d;
 #pragma unroll
                 for(int l = 0; l < D; ++l) {
-                    s.f[F_GD + l][i] = gd[l];
-                    s.f[F_GP + l][i] = (gd[l] * ui + gu[l]) * gm1;
+                    put<T>(s, F_GD + l, i, gd[l].val());
+                    put<T>(s, F_GP + l, i, (gd[l].val() * ui + gu[l].val()) * gm1);
                 }
 #pragma unroll
                 for(int c = 0; c < D; ++c)
 #pragma unroll
-                    for(int l = 0; l < D; ++l) s.f[F_GV + 3 * c + l][i] = gv[c][l] * rinv;
+                    for(int l = 0; l < D; ++l) put<T>(s, F_GV + 3 * c + l, i, gv[c][l].val() * rinv);
             }
         } else {
             const double dt = a.scal[S_DT];
-            double alpha = s.f[F_ALPHA][i];
+            const double alpha = s.f[F_ALPHA][i];
+            double dv = divv.val();
             if(a.balsara && D != 1) {
                 double rm;
                 if constexpr(TYPE == SPHGPU_DISPH) {
                     const double p_inv = gm1 / pres;
-                    divv *= p_inv;
+                    dv *= p_inv;
                     rm = rot.magnitude() * fabs(p_inv);
                 } else {
-                    divv /= rho;
-                    rm = rot.magnitude() / rho;
+                    dv /= rho_d;
+                    rm = rot.magnitude() / rho_d;
                 }
-                s.f[F_BAL][i] = fabs(divv) / (fabs(divv) + rm + 1e-4 * ci / h);
+                put<T>(s, F_BAL, i, fabs(dv) / (fabs(dv) + rm + 1e-4 * ci / h));
                 if(a.tdav) {
                     const double tau_inv = a.eps_av * ci / h;
-                    const double da = (-(alpha - a.amin) * tau_inv + fmax(-divv, 0.0) * (a.amax - alpha)) * dt;
-                    s.f[F_ALPHA][i] = alpha + da;
+                    const double da = (-(alpha - a.amin) * tau_inv + fmax(-dv, 0.0) * (a.amax - alpha)) * dt;
+                    put<T>(s, F_ALPHA, i, alpha + da);
                 }
             } else if(a.tdav) {
-                if constexpr(TYPE == SPHGPU_DISPH) divv *= gm1 / pres;
-                else divv /= rho;
+                if constexpr(TYPE == SPHGPU_DISPH) dv *= gm1 / pres;
+                else dv /= rho_d;
                 const double tau_inv = a.eps_av * ci / h;
-                const double si = fmax(-divv, 0.0);
-                s.f[F_ALPHA][i] = (alpha + dt * tau_inv * a.amin + si * dt * a.amax) / (1.0 + dt * tau_inv + si * dt);
+                const double si = fmax(-dv, 0.0);
+                put<T>(s, F_ALPHA, i, (alpha + dt * tau_inv * a.amin + si * dt * a.amax) / (1.0 + dt * tau_inv + si * dt));
             }
         }
     }
@@ -302,6 +324,7 @@ static PreArgs make_args(Ctx &c)
     PreArgs a{};
     a.list = c.list;
     a.list_r = c.list_r;
+    a.list_rf = c.list_rf;
     a.cnt = c.list_cnt;
     a.cap = c.cap;
     a.gamma = c.p.gamma;
@@ -320,50 +343,70 @@ static PreArgs make_args(Ctx &c)
     return a;
 }
 
-template <int D, int K> static void launch_initial(Ctx &c, const PreArgs &a)
+// Fields of j read by the pair sums, mirrored to FP32 before the stage
+// (alpha and the Balsara switch too: the force reads their mirrors, and
+// the stage rewrites them only when their switches are on).
+static const int kPreInputs[] = {F_V, F_V + 1, F_V + 2, F_MASS, F_ENE, F_SOUND, F_ALPHA, F_BAL};
+constexpr int kNumPreInputs = 8;
+
+template <int D, int K, class T> static void launch_initial(Ctx &c, const PreArgs &a)
 {
-    hipLaunchKernelGGL((pre_initial_kernel<D, K>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, a);
+    hipLaunchKernelGGL((pre_initial_kernel<D, K, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, a);
+}
+
+template <class T> static void initial_t(Ctx &c, const PreArgs &a)
+{
+    const int K = c.p.kernel;
+    switch(c.D) {
+    case 1: K ? launch_initial<1, 1, T>(c, a) : launch_initial<1, 0, T>(c, a); break;
+    case 2: K ? launch_initial<2, 1, T>(c, a) : launch_initial<2, 0, T>(c, a); break;
+    default: K ? launch_initial<3, 1, T>(c, a) : launch_initial<3, 0, T>(c, a); break;
+    }
 }
 
 int pre_initial(Ctx &c)
 {
+    int rc = mirror_fields(c, kPreInputs, kNumPreInputs);
+    if(rc) return rc;
     const PreArgs a = make_args(c);
-    const int K = c.p.kernel;
-    switch(c.D) {
-    case 1: K ? launch_initial<1, 1>(c, a) : launch_initial<1, 0>(c, a); break;
-    case 2: K ? launch_initial<2, 1>(c, a) : launch_initial<2, 0>(c, a); break;
-    default: K ? launch_initial<3, 1>(c, a) : launch_initial<3, 0>(c, a); break;
-    }
+    if(c.p.precision == SPHGPU_MIXED) initial_t<float>(c, a);
+    else initial_t<double>(c, a);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
 
-template <int D, int K, int TYPE> static void launch_pre(Ctx &c, const PreArgs &a, const BoxDev &box)
+template <int D, int K, int TYPE, class T> static void launch_pre(Ctx &c, const PreArgs &a, const BoxDev &box)
 {
-    hipLaunchKernelGGL((pre_kernel<D, K, TYPE>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((pre_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
-template <int D, int K> static void launch_pre_t(Ctx &c, const PreArgs &a, const BoxDev &box)
+template <int D, int K, class T> static void launch_pre_t(Ctx &c, const PreArgs &a, const BoxDev &box)
 {
     switch(c.p.sph_type) {
-    case SPHGPU_SSPH: launch_pre<D, K, SPHGPU_SSPH>(c, a, box); break;
-    case SPHGPU_DISPH: launch_pre<D, K, SPHGPU_DISPH>(c, a, box); break;
-    default: launch_pre<D, K, SPHGPU_GSPH>(c, a, box); break;
+    case SPHGPU_SSPH: launch_pre<D, K, SPHGPU_SSPH, T>(c, a, box); break;
+    case SPHGPU_DISPH: launch_pre<D, K, SPHGPU_DISPH, T>(c, a, box); break;
+    default: launch_pre<D, K, SPHGPU_GSPH, T>(c, a, box); break;
     }
 }
 
-BoxDev make_box(const Ctx &c);
+template <class T> static void pre_t(Ctx &c, const PreArgs &a, const BoxDev &box)
+{
+    const int K = c.p.kernel;
+    switch(c.D) {
+    case 1: K ? launch_pre_t<1, 1, T>(c, a, box) : launch_pre_t<1, 0, T>(c, a, box); break;
+    case 2: K ? launch_pre_t<2, 1, T>(c, a, box) : launch_pre_t<2, 0, T>(c, a, box); break;
+    default: K ? launch_pre_t<3, 1, T>(c, a, box) : launch_pre_t<3, 0, T>(c, a, box); break;
+    }
+}
 
 int pre_compute(Ctx &c)
 {
+    int rc = mirror_fields(c, kPreInputs, kNumPreInputs);
+    if(rc) return rc;
     const PreArgs a = make_args(c);
     const BoxDev box = make_box(c);
-    const int K = c.p.kernel;
-    switch(c.D) {
-    case 1: K ? launch_pre_t<1, 1>(c, a, box) : launch_pre_t<1, 0>(c, a, box); break;
-    case 2: K ? launch_pre_t<2, 1>(c, a, box) : launch_pre_t<2, 0>(c, a, box); break;
-    default: K ? launch_pre_t<3, 1>(c, a, box) : launch_pre_t<3, 0>(c, a, box); break;
-    }
+    if(c.p.precision == SPHGPU_MIXED) pre_t<float>(c, a, box);
+    else pre_t<double>(c, a, box);
     SPHGPU_HIP(hipGetLastError());
     return reduce_min_partials(c, grid_for(c.n), S_HVS);
 }

@@ -28,7 +28,8 @@ namespace sphgpu {
 
 struct SearchArgs {
     int *list;
-    double *list_r;      // |r_ij| of each candidate (gather search only)
+    double *list_r;      // |r_ij| of each candidate (gather search, FP64 mode)
+    float *list_rf;      // the same in FP32 (gather search, mixed mode)
     int *cnt;
     int cap;
     int guess;           // 1: set sml = h_guess * ratio first
@@ -123,6 +124,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     int *out = a.list + base;
     double *outr = a.list_r ? a.list_r + base : nullptr;
+    float *outrf = a.list_rf ? a.list_rf + base : nullptr;
     int count = 0;
 
     const int total = uniform(*t.num_nodes);
@@ -235,6 +237,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                         if(count < a.cap) {
                             out[(size_t)count * WAVE] = c_j[w][c];
                             if(outr) outr[(size_t)count * WAVE] = sqrt(r2);
+                            if(outrf) outrf[(size_t)count * WAVE] = (float)sqrt(r2);
                         }
                         ++count;
                     }
@@ -284,6 +287,7 @@ int search_gather(Ctx &c, bool initial)
     SearchArgs a{};
     a.list = c.list;
     a.list_r = c.list_r;
+    a.list_rf = c.list_rf;
     a.cnt = c.list_cnt;
     a.cap = c.cap;
     a.guess = 1;
@@ -298,6 +302,7 @@ int search_symmetric(Ctx &c)
     SearchArgs a{};
     a.list = c.list;
     a.list_r = nullptr;
+    a.list_rf = nullptr;
     a.cnt = c.list_cnt;
     a.cap = c.cap;
     a.guess = 0;

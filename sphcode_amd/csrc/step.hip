@@ -152,6 +152,40 @@ __global__ void energy_final_kernel(const double *partial, int nb, double *out)
     out[threadIdx.x] = t;
 }
 
+// FP32 mirrors of up to 16 FP64 fields.
+struct MirrorArgs {
+    const double *src[16];
+    float *dst[16];
+    int nf;
+};
+
+__global__ void __launch_bounds__(BLOCK) mirror_kernel(MirrorArgs m, int n)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    for(int k = 0; k < m.nf; ++k) m.dst[k][i] = (float)m.src[k][i];
+}
+
+int mirror_fields(Ctx &c, const int *fields, int nfields)
+{
+    if(c.p.precision != SPHGPU_MIXED) return SPHGPU_OK;
+    MirrorArgs m{};
+    for(int k = 0; k < nfields; ++k) {
+        const int f = fields[k];
+        if(!c.s.f[f] || !c.s.g[f]) continue;
+        m.src[m.nf] = c.s.f[f];
+        m.dst[m.nf] = c.s.g[f];
+        ++m.nf;
+        if(m.nf == 16 || k == nfields - 1) {
+            hipLaunchKernelGGL(mirror_kernel, dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, m, (int)c.n);
+            m.nf = 0;
+        }
+    }
+    if(m.nf) hipLaunchKernelGGL(mirror_kernel, dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, m, (int)c.n);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
 // ------------------------------------------------------------------ host
 BoxDev make_box(const Ctx &c)
 {

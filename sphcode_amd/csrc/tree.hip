@@ -460,7 +460,10 @@ template <int D> static int tree_build_d(Ctx &c)
     r.isrc[0] = c.s.id; r.idst[0] = c.alt.id;
     r.isrc[1] = c.s.nb; r.idst[1] = c.alt.nb;
     hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, r, c.perm, n, nf);
-    std::swap(c.s, c.alt);
+    // swap master buffers; FP32 mirrors stay (they are rewritten before use)
+    for(int f = 0; f < NF; ++f) std::swap(c.s.f[f], c.alt.f[f]);
+    std::swap(c.s.id, c.alt.id);
+    std::swap(c.s.nb, c.alt.nb);
 
     const TreeShape ts{n, G, D, c.p.tree_leaf_particle_num, c.p.tree_max_level};
     hipLaunchKernelGGL(delta_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, n, G, D, c.delta);
