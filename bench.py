@@ -8,17 +8,18 @@ synthetic initial condition (sphcode_amd.ics.evrard).  A step is one full
 Solver::integrate (timestep, predict, tree, pre-interaction, fluid force,
 gravity, correct) with the state resident in HBM.
 
-Prints one JSON line: particle-steps/s plus the roofline of the dominant
-kernel (measured live from HIP events on the context stream) and a CPU
-baseline (the oracle built with the reference's compiler flags, timed on
-this host's cores).
+`value` is the mixed-precision throughput (FP32 pair arithmetic, FP64
+kick-drift, Newton test and reductions); the FP64 throughput and the
+mixed-vs-FP64 errors on the same initial state are reported beside it.
+Also reported: the roofline of the dominant kernel (gravity), measured live
+from HIP events on the context stream, and a CPU baseline (the oracle
+built with the reference's compiler flags, timed on this host's cores).
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
-import os
 import pathlib
 import sys
 import time
@@ -38,17 +39,14 @@ def cpu_baseline(params, fields, n: int, steps: int) -> dict:
     -fopenmp) on this host: initialize untimed, then `steps` timed steps."""
     from sphcode_amd import _abi
     from sphcode_amd.engine import Engine
-    lib_path = ROOT / "oracle" / "lib" / "libsphref3_fast.so"
-    lib = C.CDLL(str(lib_path))
-    extra = {"set_threads": (C.c_int, [C.c_void_p, C.c_int])}
-    fns = _abi.bind(lib, "sphref_", extra)
+    lib = C.CDLL(str(ROOT / "oracle" / "lib" / "libsphref3_fast.so"))
+    fns = _abi.bind(lib, "sphref_", {"set_threads": (C.c_int, [C.c_void_p, C.c_int])})
     e = Engine(fns, params, name="sphref")
     threads = fns["set_threads"](None, 0)
     e.upload(fields)
     t0 = time.perf_counter()
     e.initialize()
-    t_init = time.perf_counter() - t0
-    log(f"[cpu] initialize {t_init:.2f} s with {threads} threads")
+    log(f"[cpu] initialize {time.perf_counter() - t0:.2f} s with {threads} threads")
     t0 = time.perf_counter()
     for _ in range(steps):
         e.integrate()
@@ -57,8 +55,61 @@ def cpu_baseline(params, fields, n: int, steps: int) -> dict:
     rate = n * steps / dt
     log(f"[cpu] {steps} step(s) {dt:.2f} s -> {rate:.4g} particle-steps/s")
     return {"value": rate, "unit": "particle-steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle (reference flags), Evrard N={n}: {steps} full step(s) after initialize, "
-                      f"{threads} OpenMP threads, {dt:.1f} s timed"}
+            "sample": f"oracle built with the reference's flags, Evrard N={n}, FP64: {steps} full step(s) after "
+                      f"initialize on {threads} OpenMP threads ({dt:.1f} s timed)"}
+
+
+def timed_run(params, fields, steps: int, warmup: int, label: str):
+    from sphcode_amd import GPUEngine
+    g = GPUEngine(params)
+    g.upload(fields)
+    t0 = time.perf_counter()
+    g.initialize()
+    log(f"[bench] {label}: initialize {time.perf_counter() - t0:.2f} s")
+    for k in range(warmup):
+        g.integrate()
+    g.synchronize()
+    g.reset_stats()
+    g.enable_timers(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.integrate()
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats, timers = g.stats(), g.timers()
+    g.close()
+    n = len(fields["mass"])
+    log(f"[bench] {label}: {steps} steps in {elapsed:.3f} s -> {n * steps / elapsed:.4g} particle-steps/s")
+    log(f"[bench] {label}: stage ms/step: " + ", ".join(f"{k}={v / steps:.3f}" for k, v in timers.items()))
+    log(f"[bench] {label}: stats: {stats}")
+    return elapsed, stats, timers
+
+
+def mixed_errors(params_fp64, params_mixed, fields) -> dict:
+    """Per-field error of mixed against FP64 after Solver::initialize on the
+    benchmark's initial state."""
+    import numpy as np
+
+    from sphcode_amd import GPUEngine
+    out = {}
+    res = []
+    for p in (params_fp64, params_mixed):
+        g = GPUEngine(p)
+        g.upload(fields)
+        g.initialize()
+        res.append(g.download(("dens", "sml", "pres", "acc", "phi")))
+        g.close()
+    a, b = res
+    rel = lambda x, y: float(np.max(np.abs(x - y) / np.abs(y)))
+    rms = lambda x, y: float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2)))
+    out["dens_max_rel"] = rel(b["dens"], a["dens"])
+    out["sml_max_rel"] = rel(b["sml"], a["sml"])
+    out["pres_max_rel"] = rel(b["pres"], a["pres"])
+    out["phi_max_rel"] = rel(b["phi"], a["phi"])
+    out["acc_rms_rel"] = rms(b["acc"], a["acc"])
+    out["bars"] = {"dens_max_rel": 1e-5, "acc_rms_rel": 1e-4}
+    return out
 
 
 def main() -> int:
@@ -67,51 +118,56 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 run and the error check")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_mib > 0:
+        import torch
+        src = torch.ones(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"[bench] copy calibration: {args.copy_mib} MiB copied")
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
-                                   "not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design in DESIGN.md, "
+                                   "not an implementation", "n_gpus": args.gpus}))
         return 2
 
-    from sphcode_amd import FP64, MIXED, GPUEngine, costs, ics
+    from sphcode_amd import FP64, MIXED, costs, ics
 
-    prec = MIXED if args.precision == "mixed" else FP64
-    t0 = time.perf_counter()
-    params, fields = ics.evrard(args.n, leaf=32, precision=prec)
-    log(f"[bench] initial condition N={args.n} in {time.perf_counter() - t0:.1f} s")
-    g = GPUEngine(params)
-    g.upload(fields)
-    t0 = time.perf_counter()
-    g.initialize()
-    log(f"[bench] initialize {time.perf_counter() - t0:.2f} s")
-    for k in range(args.warmup):
-        g.integrate()
-        log(f"[bench] warmup step {k + 1}/{args.warmup} t={g.time()[0]:.6g}")
-    g.synchronize()
-    g.reset_stats()
-    g.enable_timers(True)
-    g.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.integrate()
-    g.synchronize()
-    elapsed = time.perf_counter() - t0
     n = args.n
-    value = n * args.steps / elapsed
-    stats = g.stats()
-    timers = g.timers()
-    log(f"[bench] {args.steps} steps in {elapsed:.3f} s -> {value:.4g} particle-steps/s")
-    log("[bench] stage ms/step: " + ", ".join(f"{k}={v / args.steps:.3f}" for k, v in timers.items()))
-    log(f"[bench] stats: {stats}")
+    t0 = time.perf_counter()
+    p64, fields = ics.evrard(n, leaf=32, precision=FP64)
+    pmx = p64.copy()
+    pmx.precision = MIXED
+    log(f"[bench] initial condition N={n} in {time.perf_counter() - t0:.1f} s")
+    main_p = pmx if args.precision == "mixed" else p64
 
-    # Dominant kernel: the gravity walk (one launch per step).
+    errors = None
+    fp64_rate = None
+    if not args.no_compare:
+        errors = mixed_errors(p64, pmx, fields)
+        log(f"[bench] mixed vs FP64 after initialize: {errors}")
+        if args.precision == "mixed":
+            el64, _, _ = timed_run(p64, fields, args.steps, args.warmup, "fp64")
+            fp64_rate = n * args.steps / el64
+
+    elapsed, stats, timers = timed_run(main_p, fields, args.steps, args.warmup, args.precision)
+    value = n * args.steps / elapsed
+    if args.precision == "fp64":
+        fp64_rate = value
+
+    # dominant kernel: the gravity walk (one launch per step)
     grav_s = timers["gravity"] / args.steps * 1e-3
     flops = costs.gravity_flops(stats) / args.steps
     achieved = flops / grav_s / 1e12 if grav_s > 0 else 0.0
-    peak = costs.PEAK_FP64_TFLOPS if prec == FP64 else costs.PEAK_FP32_TFLOPS
+    peak = costs.PEAK_FP32_TFLOPS if args.precision == "mixed" else costs.PEAK_FP64_TFLOPS
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_gravity_evrard_{args.precision}.json"
     if tfile.exists():
@@ -121,14 +177,14 @@ def main() -> int:
             traffic = None
     roofline = {"bound": "valu", "kernel": "gravity_kernel<3>", "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                "flops_per_launch": flops, "launch_ms": grav_s * 1e3}
-    g.close()
+                "flops_per_launch": flops, "launch_ms": grav_s * 1e3,
+                "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s}
 
     cpu = None
     if not args.no_cpu:
-        cparams, cfields = ics.evrard(n, leaf=32, precision=FP64)
-        cpu = cpu_baseline(cparams, cfields, n, args.cpu_steps)
+        cpu = cpu_baseline(p64, fields, n, args.cpu_steps)
 
+    ms = elapsed / args.steps * 1e3
     out = {
         "metric": METRIC,
         "value": value,
@@ -136,15 +192,19 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else "f32+f64",
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": "evrard_collapse", "n_particles": n, "sph_type": "disph", "kernel": "wendland_c4",
                    "neighbor_number": 64, "gravity_theta": 0.5, "leaf_particle_num": 32,
                    "precision": args.precision, "balsara": True, "time_dependent_av": True},
+        "fp64_value": fp64_rate,
+        "mixed_vs_fp64": errors,
+        "pairs_per_s": {"gather": stats["gather_pairs"] / args.steps / (ms * 1e-3),
+                        "force": stats["force_pairs"] / args.steps / (ms * 1e-3)},
         "stage_ms_per_step": {k: v / args.steps for k, v in timers.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

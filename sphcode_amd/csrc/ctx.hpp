@@ -40,22 +40,15 @@ template <class T> __device__ __forceinline__ T fld(const State &s, int f, int i
     else return s.g[f][i];
 }
 
-// Per-particle sums: plain in FP64; Kahan-compensated in FP32.
+// Per-particle sums.  FP32 pair terms are compensated by accumulating them
+// in FP64: every FP32 term is exact in FP64, so the sum's error does not
+// grow with FP32 round-off times the number of terms.  On CDNA4 this is two
+// VALU operations per term (convert, add) against four for Kahan's FP32
+// compensation, at FP64 (rather than doubled-FP32) accuracy.
 template <class T> struct Acc {
-    T s = T(0);
-    __device__ __forceinline__ void add(T x) { s += x; }
-    __device__ __forceinline__ double val() const { return (double)s; }
-};
-template <> struct Acc<float> {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float x)
-    {
-        const float y = x - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double val() const { return (double)s - (double)c; }
+    double s = 0.0;
+    __device__ __forceinline__ void add(T x) { s += (double)x; }
+    __device__ __forceinline__ double val() const { return s; }
 };
 
 // Octree in depth-first pre-order; node 0 is the root.

@@ -123,7 +123,10 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kerne
 #pragma unroll
         for(int k = 0; k < D; ++k) v.xi[k] = 0.0;
     }
-    tree_walk(t, *t.num_nodes, valid, v);
+    __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
+    __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
+    const int w = uniform(threadIdx.x / WAVE);
+    tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll

@@ -131,39 +131,64 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     int node = 0;
     int64_t visited = 0, tested = 0;
     while(node < total) {
-        // ---- phase 1: collect up to LEAFCAP leaves
+        // ---- phase 1: collect up to LEAFCAP leaves.  The lanes test a
+        // window of 64 consecutive pre-order nodes at once; the visited
+        // nodes of the window are then the chain node -> next(node) ->
+        // ..., next = node+1 for an overlapping internal node and skip
+        // otherwise, followed with scalar reads of the lanes' results.
         int nleaf = 0;
         while(node < total && nleaf < LEAFCAP) {
             node = uniform(node);
-            const double4 g = t.geo[node];
-            const int4 r = t.rng[node];
-            ++visited;
-            double H = hmax;
+            const int v = node + lane;
+            int flags = 0, nxt = v + 1;
+            int4 r = make_int4(0, 0, 0, 0);
             double ks = 0.0;
-            if constexpr(IJ) {
-                ks = t.ks[node];
-                H = ks > H ? ks : H;
-            }
-            const double reach = g.w * 0.5 + H;
-            const double gcen[3] = {g.x, g.y, g.z};
-            bool overlap = true;
-#pragma unroll
-            for(int k = 0; k < D; ++k) {
-                const double dd = min_image1(box, k, gc[k] - gcen[k]);
-                overlap = overlap && fabs(dd) <= (ge[k] + reach) * (1.0 + 1e-12);
-            }
-            const int skip = r.z > node ? r.z : node + 1;
-            if(overlap && (r.w & 256)) {
-                if(lane == 0) {
-                    l_begin[w][nleaf] = r.x;
-                    l_pre[w][nleaf] = r.y - r.x;
-                    l_ks[w][nleaf] = ks;
+            if(v < total) {
+                const double4 g = t.geo[v];
+                r = t.rng[v];
+                double H = hmax;
+                if constexpr(IJ) {
+                    ks = t.ks[v];
+                    H = ks > H ? ks : H;
                 }
-                ++nleaf;
-                node = skip;
-            } else {
-                node = overlap ? node + 1 : skip;
+                const double reach = g.w * 0.5 + H;
+                const double gcen[3] = {g.x, g.y, g.z};
+                bool overlap = true;
+#pragma unroll
+                for(int k = 0; k < D; ++k) {
+                    const double dd = min_image1(box, k, gc[k] - gcen[k]);
+                    overlap = overlap && fabs(dd) <= (ge[k] + reach) * (1.0 + 1e-12);
+                }
+                const bool leaf = (r.w & 256) != 0;
+                const int skip = r.z > v ? r.z : v + 1;
+                nxt = overlap && !leaf ? v + 1 : skip;
+                flags = overlap && leaf ? 1 : 0;
             }
+            const int wend = node + WAVE < total ? node + WAVE : total;
+            int c = node;
+            while(c < wend && nleaf < LEAFCAP) {
+                const int l = c - node;
+                ++visited;
+                if(__builtin_amdgcn_readlane(flags, l)) {
+                    const int b = __builtin_amdgcn_readlane(r.x, l);
+                    const int e = __builtin_amdgcn_readlane(r.y, l);
+                    double ksl = 0.0;
+                    if constexpr(IJ) {
+                        const long long bits = __double_as_longlong(ks);
+                        const int lo32 = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), l);
+                        const int hi32 = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+                        ksl = __longlong_as_double(((long long)hi32 << 32) | (unsigned int)lo32);
+                    }
+                    if(lane == 0) {
+                        l_begin[w][nleaf] = b;
+                        l_pre[w][nleaf] = e - b;
+                        l_ks[w][nleaf] = ksl;
+                    }
+                    ++nleaf;
+                }
+                c = __builtin_amdgcn_readlane(nxt, l);
+            }
+            node = c;
         }
         wave_lds_sync();
         // prefix of leaf sizes (each lane owns two slots)
