@@ -31,6 +31,19 @@ __device__ __forceinline__ int wave_min_int(int v)
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// MI355X deals workgroups round-robin over its 8 XCDs, each with its own L2.
+// Renumbering the launch so that XCD x runs one contiguous range of
+// logical blocks keeps spatially adjacent particles (adjacent in Morton
+// order) and their neighbours in the same L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int per = nb / NUM_XCD, rem = nb % NUM_XCD;
+    const int x = b % NUM_XCD, k = b / NUM_XCD;
+    return x * per + (x < rem ? x : rem) + k;
+}
+__device__ __forceinline__ int xcd_block() { return uniform(xcd_block((int)blockIdx.x, (int)gridDim.x)); }
+
 template <class T> __device__ __forceinline__ T wave_sum(T v)
 {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

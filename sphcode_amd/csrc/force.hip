@@ -6,8 +6,8 @@
 // acc and dene are assigned, not accumulated, as in the reference.
 //
 // T is the pair arithmetic (double, or float in mixed mode: j fields from
-// FP32 mirrors, separations taken in FP64 then rounded, Kahan-compensated
-// sums; the pair cut is tested on the FP64 squared distance).
+// FP32 mirrors, separations taken in FP64 then rounded, FP32 terms
+// accumulated in FP64; the pair cut is tested on the FP64 squared distance).
 #include "ctx.hpp"
 
 namespace sphgpu {
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool FP64 = sizeof(T) == 8;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     const bool valid = i < n;
     int64_t pairs = 0;
     if(valid) {

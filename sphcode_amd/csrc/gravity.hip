@@ -12,7 +12,7 @@
 // T is the interaction arithmetic.  The opening test always runs in FP64,
 // so both precisions take the same interaction lists; in mixed mode the
 // separations are taken in FP64 relative to particle i, rounded to FP32,
-// and the sums are Kahan-compensated.
+// and the FP32 terms are accumulated in FP64 (Acc in ctx.hpp).
 #include "walk.hpp"
 
 namespace sphgpu {
@@ -109,7 +109,7 @@ template <int D, class T> struct GravityVisitor {
 
 template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     const bool valid = i < n;
     GravityVisitor<D, T> v(t, box, s);
     v.G = a.G;

@@ -19,8 +19,9 @@
 //
 // T is the pair arithmetic: double, or float for the mixed mode, where
 // pair terms are evaluated in FP32 relative to particle i (separations
-// taken in FP64, then rounded), summed with Kahan compensation, and the
-// Newton update, its convergence test and every stored result stay FP64.
+// taken in FP64, then rounded) and accumulated in FP64 (Acc in ctx.hpp),
+// and the Newton update, its convergence test and every stored result
+// stay FP64.
 #include "ctx.hpp"
 
 namespace sphgpu {
@@ -83,7 +84,7 @@ template <int D, int K, class T>
 __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, int n, PreArgs a)
 {
     using Kern = typename KernelOf<D, K, T>::type;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool NUMBER = TYPE == SPHGPU_DISPH;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     const bool valid = i < n;
     double hvs = 1.7976931348623157e308;
     int64_t iters = 0, fails = 0, trunc = 0, pairs = 0;

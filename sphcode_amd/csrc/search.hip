@@ -89,7 +89,12 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     const int lane = wave_lane();
     const int ngroups = uniform(*num_groups);
     const int nwaves = gridDim.x * SW;
-    for(int grp = blockIdx.x * SW + w; grp < ngroups; grp += nwaves) {
+    // each wave takes a contiguous run of groups; with the XCD-aware block
+    // order every XCD sweeps one contiguous stretch of the Morton curve
+    const int chunk = (ngroups + nwaves - 1) / nwaves;
+    const int g0 = (xcd_block() * SW + w) * chunk;
+    const int g1 = g0 + chunk < ngroups ? g0 + chunk : ngroups;
+    for(int grp = g0; grp < g1; ++grp) {
     const int2 gr = make_int2(uniform(groups[grp].x), uniform(groups[grp].y));
     const int i = gr.x + lane;
     const bool valid = i < gr.y;
