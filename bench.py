@@ -86,6 +86,54 @@ def timed_run(params, fields, steps: int, warmup: int, label: str):
     return elapsed, stats, timers
 
 
+def adjoint_run(n_side: int, steps: int, dt: float) -> dict:
+    """Differentiable rollout on the 3D periodic box (n_side^3 particles,
+    SSPH cubic spline, 50 neighbours, FP64): time of the plain forward
+    (Solver::initialize + `steps` fixed-dt steps), of the checkpointed
+    forward, and of the adjoint of L = sum m |v_K|^2."""
+    import numpy as np
+    from sphcode_amd import GPUEngine, ics
+    p, f = ics.rollout(n_side=n_side)
+    n = n_side ** 3
+    g = GPUEngine(p)
+
+    def plain():
+        g.upload(f)
+        g.initialize()
+        for _ in range(steps):
+            g.integrate(dt)
+        g.synchronize()
+
+    plain()  # warm-up (allocation, code load)
+    t0 = time.perf_counter()
+    plain()
+    t_plain = time.perf_counter() - t0
+    g.upload(f)
+    g.rollout(steps, dt)  # warm-up of the checkpoint buffers
+    g.upload(f)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.rollout(steps, dt)
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    out = g.download(("mass", "vel"))
+    seed = 2.0 * out["mass"][:, None] * out["vel"]
+    g.rollout_adjoint(dl_dv=seed)  # warm-up
+    g.upload(f)
+    g.rollout(steps, dt)
+    g.synchronize()
+    t0 = time.perf_counter()
+    gx, gv, gu = g.rollout_adjoint(dl_dv=seed)
+    t_adj = time.perf_counter() - t0
+    g.close()
+    log(f"[bench] adjoint: N={n} K={steps} forward {t_plain * 1e3:.1f} ms, checkpointed forward "
+        f"{t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms")
+    return {"workload": "periodic_box_rollout", "n_particles": n, "steps": steps, "dt": dt, "precision": "fp64",
+            "loss": "sum m |v_K|^2", "forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
+            "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain,
+            "grad_norms": [float(np.linalg.norm(gx)), float(np.linalg.norm(gv)), float(np.linalg.norm(gu))]}
+
+
 def mixed_errors(params_fp64, params_mixed, fields) -> dict:
     """Per-field error of mixed against FP64 after Solver::initialize on the
     benchmark's initial state."""
@@ -122,6 +170,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 run and the error check")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout timing")
+    ap.add_argument("--adjoint-side", type=int, default=64)
+    ap.add_argument("--adjoint-steps", type=int, default=16)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -180,6 +231,10 @@ def main() -> int:
                 "flops_per_launch": flops, "launch_ms": grav_s * 1e3,
                 "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s}
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_run(args.adjoint_side, args.adjoint_steps, 1e-3)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p64, fields, n, args.cpu_steps)
@@ -207,6 +262,7 @@ def main() -> int:
                         "force": stats["force_pairs"] / args.steps / (ms * 1e-3)},
         "stage_ms_per_step": {k: v / args.steps for k, v in timers.items()},
         "roofline": roofline,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

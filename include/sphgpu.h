@@ -237,6 +237,31 @@ int sphgpu_tree_info(void *ctx, int64_t *num_nodes, int32_t *level, int32_t *cou
 
 int sphgpu_synchronize(void *ctx);
 
+/*
+ * Differentiable rollout.  No counterpart in the reference: this is the
+ * gradient of a K-step trajectory with respect to its initial state.
+ *
+ * sphgpu_rollout runs Solver::initialize on the uploaded state and then
+ * `steps` integrate() steps with dt fixed (as sphgpu_integrate(ctx, dt)),
+ * keeping a checkpoint of the state at the start of every step.  The
+ * context ends at the final state.
+ *
+ * sphgpu_rollout_adjoint takes the gradient of a scalar loss with respect
+ * to the final positions, velocities and specific energies (caller order,
+ * n*dim / n*dim / n doubles; NULL = zero) and returns its gradient with
+ * respect to the initial ones (NULL outputs are skipped).  Each step's
+ * force evaluation is recomputed from its checkpoint and differentiated
+ * with its neighbour lists held fixed; h enters through the implicit
+ * function theorem on rho(h) h^D = m N_ngb / A_D.  Supported: SSPH with
+ * iterative h, artificial viscosity with or without the Balsara switch,
+ * periodic or open boundaries, FP64 precision; no time-dependent
+ * viscosity, conductivity or gravity (SPHGPU_E_INVALID otherwise).  The
+ * context is left at the final state of the rollout.
+ */
+int sphgpu_rollout(void *ctx, int steps, double dt);
+int sphgpu_rollout_adjoint(void *ctx, const double *dl_dx, const double *dl_dv, const double *dl_du,
+                           double *dx0, double *dv0, double *du0);
+
 #ifdef __cplusplus
 }
 #endif

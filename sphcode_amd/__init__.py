@@ -10,6 +10,8 @@ from __future__ import annotations
 import ctypes as C
 import pathlib
 
+import numpy as np
+
 from . import _abi
 from ._abi import (CUBIC_SPLINE, DISPH, FP64, GSPH, MIXED, SSPH, WENDLAND,  # noqa: F401
                    Params, default_params)
@@ -49,3 +51,29 @@ class GPUEngine(Engine):
 
     def synchronize(self) -> None:
         self._call("synchronize")
+
+    # ---------------------------------------------------- differentiable rollout
+    def rollout(self, steps: int, dt: float) -> None:
+        """Solver::initialize, then ``steps`` steps of fixed ``dt``, with
+        per-step checkpoints for :meth:`rollout_adjoint`."""
+        self._call("rollout", C.c_int(int(steps)), C.c_double(float(dt)))
+
+    def rollout_adjoint(self, dl_dx=None, dl_dv=None, dl_du=None):
+        """Gradient of a loss with respect to the initial (pos, vel, ene),
+        given its gradient with respect to the final ones (caller order;
+        None means zero)."""
+        n, d = self.n, self.dim
+
+        def arg(a, shape):
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+            return a, a.ctypes.data_as(_abi._DP)
+
+        sx, px = arg(dl_dx, (n, d))
+        sv, pv = arg(dl_dv, (n, d))
+        su, pu = arg(dl_du, (n,))
+        gx, gv, gu = np.zeros((n, d)), np.zeros((n, d)), np.zeros(n)
+        self._call("rollout_adjoint", px, pv, pu, gx.ctypes.data_as(_abi._DP), gv.ctypes.data_as(_abi._DP),
+                   gu.ctypes.data_as(_abi._DP))
+        return gx, gv, gu

@@ -186,12 +186,16 @@ _PREFIXED = {
     "tree_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), _ip, _ip, _dp, _dp, _dp, _dp, C.c_int64]),
 }
 
+_DP = C.POINTER(C.c_double)
+
 GPU_ONLY = {
     "version": (C.c_int, []),
     "params_default": (C.c_int, [C.POINTER(Params)]),
     "get_timers": (C.c_int, [C.c_void_p, C.POINTER(Timers)]),
     "enable_timers": (C.c_int, [C.c_void_p, C.c_int]),
     "synchronize": (C.c_int, [C.c_void_p]),
+    "rollout": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "rollout_adjoint": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP, _DP, _DP]),
 }
 
 

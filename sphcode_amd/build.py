@@ -30,7 +30,7 @@ HOST_SOURCES = ["json.cpp", "parameters.cpp", "simulation.cpp", "modules.cpp", "
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-I", str(ROOT / "include"), "-I", str(PKG / "host" / "include")]
 
-SOURCES = ["api.hip", "tree.hip", "search.hip", "pre.hip", "force.hip", "gravity.hip", "step.hip"]
+SOURCES = ["api.hip", "tree.hip", "search.hip", "pre.hip", "force.hip", "gravity.hip", "step.hip", "adjoint.hip"]
 
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",

@@ -285,6 +285,7 @@ int sphgpu_destroy(void *ptr)
     Ctx *c = as_ctx(ptr);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    adjoint_free(*c);
     free_buffers(*c);
     if(c->scal) (void)hipFree(c->scal);
     if(c->counters) (void)hipFree(c->counters);
@@ -524,6 +525,107 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
             c.timer_ms[k] += ms;
         }
     }
+    return SPHGPU_OK;
+}
+
+// One full step with a fixed dt, without timers (the rollout and its
+// recomputation use exactly this sequence).
+static int rollout_step(Ctx &c, double dt)
+{
+    int rc;
+    hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, dt);
+    if((rc = step_predict(c))) return rc;
+    if((rc = stage_tree(c))) return rc;
+    if((rc = stage_pre(c))) return rc;
+    if((rc = stage_force(c))) return rc;
+    if((rc = gravity_compute(c))) return rc;
+    return step_correct(c);
+}
+
+// The force evaluation of step `slot - 1` (slot >= 1), or of
+// Solver::initialize (slot 0), recomputed from its checkpoint.
+static int rollout_recompute(Ctx &c, int slot)
+{
+    int rc;
+    if((rc = adjoint_restore(c, slot))) return rc;
+    if(slot == 0) {
+        c.first_pre = true;
+        if((rc = step_init_fields(c))) return rc;
+    } else {
+        c.first_pre = false;
+        hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, c.ro.dt);
+        if((rc = step_predict(c))) return rc;
+    }
+    if((rc = stage_tree(c))) return rc;
+    if((rc = stage_pre(c))) return rc;
+    return stage_force(c);
+}
+
+static int rollout_supported(const Ctx &c)
+{
+    const sphgpu_params &p = c.p;
+    if(p.sph_type != SPHGPU_SSPH) return set_error(SPHGPU_E_INVALID, "rollout adjoint: SSPH only");
+    if(!p.iterative_sml) return set_error(SPHGPU_E_INVALID, "rollout adjoint: needs iterativeSmoothingLength");
+    if(p.av_use_time_dependent) return set_error(SPHGPU_E_INVALID, "rollout adjoint: time-dependent viscosity is not differentiated");
+    if(p.ac_is_valid) return set_error(SPHGPU_E_INVALID, "rollout adjoint: artificial conductivity is not differentiated");
+    if(p.gravity_is_valid) return set_error(SPHGPU_E_INVALID, "rollout adjoint: gravity is not differentiated");
+    if(p.precision != SPHGPU_FP64) return set_error(SPHGPU_E_INVALID, "rollout adjoint: FP64 precision only");
+    return SPHGPU_OK;
+}
+
+int sphgpu_rollout(void *ptr, int steps, double dt)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    int rc;
+    if(steps < 1) return set_error(SPHGPU_E_INVALID, "rollout: steps must be >= 1");
+    if(!(dt > 0.0)) return set_error(SPHGPU_E_INVALID, "rollout: dt must be > 0");
+    if((rc = rollout_supported(c))) return rc;
+    if((rc = adjoint_alloc(c, steps))) return rc;
+    c.ro.dt = dt;
+    c.ro.t0 = c.time;
+    if((rc = adjoint_save(c, 0))) return rc;
+    c.first_pre = true;
+    if((rc = step_init_fields(c))) return rc;
+    if((rc = stage_tree(c))) return rc;
+    if((rc = stage_pre(c))) return rc;
+    if((rc = stage_force(c))) return rc;
+    if((rc = gravity_compute(c))) return rc;
+    for(int k = 0; k < steps; ++k) {
+        if((rc = adjoint_save(c, k + 1))) return rc;
+        if((rc = rollout_step(c, dt))) return rc;
+    }
+    if((rc = adjoint_save(c, steps + 1))) return rc;
+    SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if((rc = check_flags(c, "rollout"))) return rc;
+    c.dt = dt;
+    c.h_per_v_sig = c.scal_host[S_HVS];
+    c.time = c.ro.t0 + steps * dt;
+    c.steps += steps;
+    c.ro.steps = steps;
+    return SPHGPU_OK;
+}
+
+int sphgpu_rollout_adjoint(void *ptr, const double *dl_dx, const double *dl_dv, const double *dl_du, double *dx0,
+                           double *dv0, double *du0)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    int rc;
+    if(c.ro.steps < 1 || c.ro.n != c.n) return set_error(SPHGPU_E_STATE, "rollout adjoint: no stored rollout");
+    const int K = c.ro.steps;
+    if((rc = adjoint_seed(c, dl_dx, dl_dv, dl_du))) return rc;
+    for(int k = K; k >= 0; --k) {
+        if((rc = rollout_recompute(c, k))) return rc;
+        if((rc = adjoint_step(c, k > 0))) return rc;
+    }
+    if((rc = check_flags(c, "rollout_adjoint"))) return rc;
+    if((rc = adjoint_fetch(c, dx0, dv0, du0))) return rc;
+    // leave the context at the final state of the rollout, with its tree
+    if((rc = adjoint_restore(c, K + 1))) return rc;
+    c.first_pre = false;
+    if((rc = stage_tree(c))) return rc;
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
     return SPHGPU_OK;
 }
 

@@ -124,6 +124,20 @@ struct Ctx {
     double time = 0.0, dt = 0.0, h_per_v_sig = 0.0;
     bool uploaded = false, first_pre = true, tree_ready = false;
 
+    // differentiable rollout: per-step checkpoints (device order, with the
+    // id permutation of that moment) and adjoint buffers
+    struct Rollout {
+        int steps = 0;          // K of the stored rollout (0: none)
+        double dt = 0.0;
+        double t0 = 0.0;
+        int64_t n = 0;
+        double *ck = nullptr;   // (K + 2) slots x NCK fields x n
+        int *ck_id = nullptr;   // (K + 2) slots x n
+        double *adj = nullptr;  // adjoint of the state, caller order
+        double *work = nullptr; // per-stage adjoint arrays, device order
+        int slots = 0;          // allocated slots
+    } ro;
+
     // stage timers
     bool timers_on = false;
     hipEvent_t ev[16];
@@ -170,5 +184,15 @@ int step_energy(Ctx &c, double *out);
 // FP32 mirrors of the listed fields (mixed precision only; no-op in FP64)
 int mirror_fields(Ctx &c, const int *fields, int nfields);
 int reduce_min_partials(Ctx &c, int nblocks, int slot);
+// adjoint.hip: vector-Jacobian product of one force evaluation
+// (pre-interaction + fluid force at the current state and lists) and the
+// kick-drift around it; kick = false for the evaluation of Solver::initialize
+int adjoint_step(Ctx &c, bool kick);
+int adjoint_alloc(Ctx &c, int steps);
+void adjoint_free(Ctx &c);
+int adjoint_save(Ctx &c, int slot);
+int adjoint_restore(Ctx &c, int slot);
+int adjoint_seed(Ctx &c, const double *dx, const double *dv, const double *du);
+int adjoint_fetch(Ctx &c, double *dx, double *dv, double *du);
 
 }  // namespace sphgpu
