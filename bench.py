@@ -86,14 +86,14 @@ def timed_run(params, fields, steps: int, warmup: int, label: str):
     return elapsed, stats, timers
 
 
-def adjoint_run(n_side: int, steps: int, dt: float) -> dict:
+def adjoint_run(n_side: int, steps: int, dt: float, precision: int) -> dict:
     """Differentiable rollout on the 3D periodic box (n_side^3 particles,
-    SSPH cubic spline, 50 neighbours, FP64): time of the plain forward
+    SSPH cubic spline, 50 neighbours): time of the plain forward
     (Solver::initialize + `steps` fixed-dt steps), of the checkpointed
     forward, and of the adjoint of L = sum m |v_K|^2."""
     import numpy as np
-    from sphcode_amd import GPUEngine, ics
-    p, f = ics.rollout(n_side=n_side)
+    from sphcode_amd import FP64, GPUEngine, ics
+    p, f = ics.rollout(n_side=n_side, precision=precision)
     n = n_side ** 3
     g = GPUEngine(p)
 
@@ -126,12 +126,12 @@ def adjoint_run(n_side: int, steps: int, dt: float) -> dict:
     gx, gv, gu = g.rollout_adjoint(dl_dv=seed)
     t_adj = time.perf_counter() - t0
     g.close()
-    log(f"[bench] adjoint: N={n} K={steps} forward {t_plain * 1e3:.1f} ms, checkpointed forward "
+    label = "fp64" if precision == FP64 else "mixed"
+    log(f"[bench] adjoint {label}: N={n} K={steps} forward {t_plain * 1e3:.1f} ms, checkpointed forward "
         f"{t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms")
-    return {"workload": "periodic_box_rollout", "n_particles": n, "steps": steps, "dt": dt, "precision": "fp64",
+    return {"workload": "periodic_box_rollout", "n_particles": n, "steps": steps, "dt": dt, "precision": label,
             "loss": "sum m |v_K|^2", "forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
-            "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain,
-            "grad_norms": [float(np.linalg.norm(gx)), float(np.linalg.norm(gv)), float(np.linalg.norm(gu))]}
+            "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain}, (gx, gv, gu)
 
 
 def mixed_errors(params_fp64, params_mixed, fields) -> dict:
@@ -233,7 +233,13 @@ def main() -> int:
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_run(args.adjoint_side, args.adjoint_steps, 1e-3)
+        import numpy as np
+        adj64, g64 = adjoint_run(args.adjoint_side, args.adjoint_steps, 1e-3, FP64)
+        adjoint, gmx = adjoint_run(args.adjoint_side, args.adjoint_steps, 1e-3, MIXED)
+        adjoint["fp64"] = adj64
+        adjoint["grad_rel_err_vs_fp64"] = {
+            name: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for name, a, b in zip(("pos", "vel", "ene"), gmx, g64)}
+        log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}")
 
     cpu = None
     if not args.no_cpu:

@@ -60,9 +60,9 @@ extern "C" {
 
 /* Arithmetic of the pair sums. */
 #define SPHGPU_FP64 0   /* all FP64, as real=double (include/defines.hpp:6)   */
-#define SPHGPU_MIXED 1  /* FP32 pair arithmetic relative to particle i with
-                           compensated per-particle sums; FP64 kick-drift,
-                           Newton convergence test and global reductions     */
+#define SPHGPU_MIXED 1  /* FP32 pair arithmetic relative to particle i, FP32
+                           terms summed in FP64; FP64 kick-drift, Newton
+                           update and test, and global reductions            */
 
 /*
  * sphgpu_params mirrors SPHParameters (include/parameters.hpp:20-79) field by
@@ -254,7 +254,8 @@ int sphgpu_synchronize(void *ctx);
  * with its neighbour lists held fixed; h enters through the implicit
  * function theorem on rho(h) h^D = m N_ngb / A_D.  Supported: SSPH with
  * iterative h, artificial viscosity with or without the Balsara switch,
- * periodic or open boundaries, FP64 precision; no time-dependent
+ * periodic or open boundaries, FP64 or mixed precision (the adjoint's pair
+ * arithmetic then is FP32 with FP64 accumulation); no time-dependent
  * viscosity, conductivity or gravity (SPHGPU_E_INVALID otherwise).  The
  * context is left at the final state of the rollout.
  */

@@ -24,7 +24,8 @@
 //   2. gather sums  -> chain through P, Omega, Balsara, h (IFT) to seeds on
 //                      the sums rho, d rho/dh, div v, curl v
 //   3. gather pairs -> adjoints of x and v through those sums
-// The arithmetic is FP64 throughout.
+// The pair arithmetic follows the context's precision (FP64, or FP32 with
+// FP64 accumulation); the per-particle chain rule and the IFT are FP64.
 #include <vector>
 
 #include "ctx.hpp"
@@ -101,104 +102,125 @@ template <int D> __global__ void __launch_bounds__(BLOCK) adj_seed_kernel(State 
     W(a, W_DUF, i) = ADJ<D>(a, L::DU, id) + half * ADJ<D>(a, L::U, id);
 }
 
-// pass 1: force pairs (src/fluid_force.cpp:26-106)
-template <int D, int K> __global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, int n, AdjArgs a)
+// Pair arithmetic of the passes: T = double, or float in mixed precision
+// (separations taken in FP64 and rounded, fields of j from the FP32
+// mirrors, terms accumulated in FP64 by Acc<T>); the pair-set tests are
+// those of the forward stages in the same precision.
+template <class T> __device__ __forceinline__ T pair_r(double r2)
 {
+    if constexpr(sizeof(T) == 8) return sqrt(r2);
+    else return (float)sqrt(r2);
+}
+
+// pass 1: force pairs (src/fluid_force.cpp:26-106)
+template <int D, int K, class T>
+__global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, int n, AdjArgs a)
+{
+    constexpr bool FP64 = sizeof(T) == 8;
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
-    double xi[D], vi[D], afi[D];
+    double xi[D];
+    T vi[D], afi[D];
 #pragma unroll
     for(int k = 0; k < D; ++k) {
         xi[k] = s.f[F_X + k][i];
-        vi[k] = s.f[F_V + k][i];
-        afi[k] = W(a, W_AF + k, i);
+        vi[k] = (T)s.f[F_V + k][i];
+        afi[k] = (T)W(a, W_AF + k, i);
     }
-    const double hi = s.f[F_SML][i], rhoi = s.f[F_DENS][i], pi_ = s.f[F_PRES][i], omi = s.f[F_GRADH][i];
-    const double bi = s.f[F_BAL][i], ci = s.f[F_SOUND][i], ali = s.f[F_ALPHA][i], mi = s.f[F_MASS][i];
-    const double dufi = W(a, W_DUF, i);
-    const double Ai = pi_ * omi / (rhoi * rhoi);
+    const double hi_d = s.f[F_SML][i];
+    const T hi = (T)hi_d, rhoi = (T)s.f[F_DENS][i], pi_ = (T)s.f[F_PRES][i], omi = (T)s.f[F_GRADH][i];
+    const T bi = (T)s.f[F_BAL][i], ci = (T)s.f[F_SOUND][i], ali = (T)s.f[F_ALPHA][i], mi = (T)s.f[F_MASS][i];
+    const T dufi = (T)W(a, W_DUF, i);
+    const T Ai = pi_ * omi / (rhoi * rhoi);
 
-    double xb[D], vb[D];
-#pragma unroll
-    for(int k = 0; k < D; ++k) xb[k] = vb[k] = 0.0;
-    double hb = 0.0, rhob = 0.0, Ab = 0.0, Bb = 0.0, cb = 0.0;
-
+    Acc<T> xb[D], vb[D], hb, rhob, Ab, Bb, cb;
     for(int kk = 0; kk < cnt; ++kk) {
         const int j = a.list[base + (size_t)kk * WAVE];
-        double xj[D], d[D];
+        double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-        const double r2 = sep2<D>(box, xi, xj, d);
-        const double hj = s.f[F_SML][j];
-        const double hmax = hi < hj ? hj : hi;
-        const double r = sqrt(r2);
-        if(r >= hmax || r == 0.0) continue;
-        const double rhoj = s.f[F_DENS][j], pj = s.f[F_PRES][j], omj = s.f[F_GRADH][j];
-        const double bj = s.f[F_BAL][j], cj = s.f[F_SOUND][j], alj = s.f[F_ALPHA][j], mj = s.f[F_MASS][j];
-        const double dufj = W(a, W_DUF, j);
-        const double Aj = pj * omj / (rhoj * rhoj);
-        const KernelDerivs<double> ki = kernel_derivs<D, K, double>(r, hi);
-        const KernelDerivs<double> kj = kernel_derivs<D, K, double>(r, hj);
-        const double gi = ki.f, gj = kj.f, gb = 0.5 * (gi + gj);
-        double v[D], beta[D];
-        double sv = 0.0, b = 0.0;
+        const double r2 = sep2<D>(box, xi, xj, dd);
+        const double hj_d = s.f[F_SML][j];
+        const double hmax = hi_d < hj_d ? hj_d : hi_d;
+        T r;
+        if constexpr(FP64) {
+            r = sqrt(r2);
+            if(r >= hmax || r == 0.0) continue;
+        } else {
+            if(!(r2 < hmax * hmax) || r2 == 0.0) continue;
+            r = sqrtf((float)r2);
+        }
+        T d[D];
+#pragma unroll
+        for(int k = 0; k < D; ++k) d[k] = (T)dd[k];
+        const T hj = fld<T>(s, F_SML, j);
+        const T rhoj = fld<T>(s, F_DENS, j), pj = fld<T>(s, F_PRES, j), omj = fld<T>(s, F_GRADH, j);
+        const T bj = fld<T>(s, F_BAL, j), cj = fld<T>(s, F_SOUND, j), alj = fld<T>(s, F_ALPHA, j);
+        const T mj = fld<T>(s, F_MASS, j);
+        const T dufj = (T)W(a, W_DUF, j);
+        const T Aj = pj * omj / (rhoj * rhoj);
+        const KernelDerivs<T> ki = kernel_derivs<D, K, T>(r, hi);
+        const KernelDerivs<T> kj = kernel_derivs<D, K, T>(r, hj);
+        const T gi = ki.f, gj = kj.f, gb = T(0.5) * (gi + gj);
+        T v[D], beta[D];
+        T sv = T(0), b = T(0);
 #pragma unroll
         for(int k = 0; k < D; ++k) {
-            v[k] = vi[k] - s.f[F_V + k][j];
-            beta[k] = mi * W(a, W_AF + k, j) - mj * afi[k];
+            v[k] = vi[k] - fld<T>(s, F_V + k, j);
+            beta[k] = mi * (T)W(a, W_AF + k, j) - mj * afi[k];
             sv += v[k] * d[k];
             b += beta[k] * d[k];
         }
-        const double psii = mj * dufi, psij = mi * dufj, psis = psii + psij;
+        const T psii = mj * dufi, psij = mi * dufj, psis = psii + psij;
         // Monaghan viscosity Pi = K (C - 3 w) w for w = v.r / r < 0
-        double Pi = 0.0, Kv = 0.0, w = 0.0;
-        const double Cs = ci + cj, rhos = rhoi + rhoj, al2 = ali + alj;
-        if(sv < 0.0) {
+        T Pi = T(0), Kv = T(0), w = T(0);
+        const T Cs = ci + cj, rhos = rhoi + rhoj, al2 = ali + alj;
+        if(sv < T(0)) {
             w = sv / r;
-            Kv = -0.25 * (bi + bj) * al2 / rhos;
-            Pi = Kv * (Cs - 3.0 * w) * w;
+            Kv = T(-0.25) * (bi + bj) * al2 / rhos;
+            Pi = Kv * (Cs - T(3) * w) * w;
         }
-        const double Phi = Ai * gi + Aj * gj + Pi * gb;
-        const double EPi = (b + 0.5 * sv * psis) * gb;
-        const double Egi = b * (Ai + 0.5 * Pi) + sv * (psii * Ai + 0.25 * psis * Pi);
-        const double Egj = b * (Aj + 0.5 * Pi) + sv * (psij * Aj + 0.25 * psis * Pi);
-        double Es = psii * Ai * gi + psij * Aj * gj + 0.5 * psis * Pi * gb;
-        double Er = Egi * ki.fr + Egj * kj.fr;
-        if(sv < 0.0) {
-            const double Piw = Kv * (Cs - 6.0 * w);
+        const T Phi = Ai * gi + Aj * gj + Pi * gb;
+        const T EPi = (b + T(0.5) * sv * psis) * gb;
+        const T Egi = b * (Ai + T(0.5) * Pi) + sv * (psii * Ai + T(0.25) * psis * Pi);
+        const T Egj = b * (Aj + T(0.5) * Pi) + sv * (psij * Aj + T(0.25) * psis * Pi);
+        T Es = psii * Ai * gi + psij * Aj * gj + T(0.5) * psis * Pi * gb;
+        T Er = Egi * ki.fr + Egj * kj.fr;
+        if(sv < T(0)) {
+            const T Piw = Kv * (Cs - T(6) * w);
             Es += EPi * Piw / r;
             Er -= EPi * Piw * w / r;
-            rhob -= EPi * Pi / rhos;
-            cb += EPi * Kv * w;
-            Bb += EPi * (Cs - 3.0 * w) * w * (-0.25 * al2 / rhos);
+            rhob.add(-(EPi * Pi / rhos));
+            cb.add(EPi * Kv * w);
+            Bb.add(EPi * (Cs - T(3) * w) * w * (T(-0.25) * al2 / rhos));
         }
-        const double er = Er / r;
+        const T er = Er / r;
 #pragma unroll
         for(int k = 0; k < D; ++k) {
-            xb[k] += beta[k] * Phi + v[k] * Es + d[k] * er;
-            vb[k] += d[k] * Es;
+            xb[k].add(beta[k] * Phi + v[k] * Es + d[k] * er);
+            vb[k].add(d[k] * Es);
         }
-        hb += Egi * ki.fh;
-        Ab += (b + sv * psii) * gi;
+        hb.add(Egi * ki.fh);
+        Ab.add((b + sv * psii) * gi);
     }
 #pragma unroll
     for(int k = 0; k < D; ++k) {
-        W(a, W_XB + k, i) = xb[k];
-        W(a, W_VB + k, i) = vb[k];
+        W(a, W_XB + k, i) = xb[k].val();
+        W(a, W_VB + k, i) = vb[k].val();
     }
-    W(a, W_HB, i) = hb;
-    W(a, W_RHOB, i) = rhob;
-    W(a, W_AB, i) = Ab;
-    W(a, W_BB, i) = Bb;
-    W(a, W_CB, i) = cb;
+    W(a, W_HB, i) = hb.val();
+    W(a, W_RHOB, i) = rhob.val();
+    W(a, W_AB, i) = Ab.val();
+    W(a, W_BB, i) = Bb.val();
+    W(a, W_CB, i) = cb.val();
 }
 
 // v x r for D = 3 (3 components) and D = 2 (z only, stored in slot 2)
-template <int D> __device__ __forceinline__ void cross_vr(const double v[D], const double d[D], double o[3])
+template <int D, class T> __device__ __forceinline__ void cross_vr(const T v[D], const T d[D], T o[3])
 {
-    o[0] = o[1] = o[2] = 0.0;
+    o[0] = o[1] = o[2] = T(0);
     if constexpr(D == 3) {
         o[0] = v[1] * d[2] - v[2] * d[1];
         o[1] = v[2] * d[0] - v[0] * d[2];
@@ -209,52 +231,58 @@ template <int D> __device__ __forceinline__ void cross_vr(const double v[D], con
 }
 
 // pass 2: gather sums of particle i and the chain rule through its
-// derived quantities (src/pre_interaction.cpp:66-169)
-template <int D, int K> __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int n, AdjArgs a)
+// derived quantities (src/pre_interaction.cpp:66-169), the latter in FP64
+template <int D, int K, class T>
+__global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int n, AdjArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
-    double xi[D], vi[D];
+    double xi[D];
+    T vi[D];
 #pragma unroll
     for(int k = 0; k < D; ++k) {
         xi[k] = s.f[F_X + k][i];
-        vi[k] = s.f[F_V + k][i];
+        vi[k] = (T)s.f[F_V + k][i];
     }
     const double h = s.f[F_SML][i];
-    double rho = 0.0, rhop = 0.0, rhopp = 0.0, Dv = 0.0, Dvh = 0.0;
-    double Rv[3] = {0.0, 0.0, 0.0}, Rvh[3] = {0.0, 0.0, 0.0};
+    const T ht = (T)h;
+    Acc<T> rho_a, rhop_a, rhopp_a, Dv_a, Dvh_a, Rv_a[3], Rvh_a[3];
     for(int kk = 0; kk < cnt; ++kk) {
         const int j = a.list[base + (size_t)kk * WAVE];
-        double xj[D], d[D];
+        double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-        const double r = sqrt(sep2<D>(box, xi, xj, d));
-        if(!(r < h)) continue;
-        const double mj = s.f[F_MASS][j];
-        const KernelDerivs<double> kd = kernel_derivs<D, K, double>(r, h);
-        rho += mj * kd.w;
-        rhop += mj * kd.wh;
-        rhopp += mj * kd.whh;
+        const T r = pair_r<T>(sep2<D>(box, xi, xj, dd));
+        if(!(r < ht)) continue;
+        const T mj = fld<T>(s, F_MASS, j);
+        const KernelDerivs<T> kd = kernel_derivs<D, K, T>(r, ht);
+        rho_a.add(mj * kd.w);
+        rhop_a.add(mj * kd.wh);
+        rhopp_a.add(mj * kd.whh);
         if(j != i) {
-            double v[D], sv = 0.0;
+            T v[D], d[D], sv = T(0);
 #pragma unroll
             for(int k = 0; k < D; ++k) {
-                v[k] = vi[k] - s.f[F_V + k][j];
+                d[k] = (T)dd[k];
+                v[k] = vi[k] - fld<T>(s, F_V + k, j);
                 sv += v[k] * d[k];
             }
-            double cr[3];
-            cross_vr<D>(v, d, cr);
-            Dv += mj * sv * kd.f;
-            Dvh += mj * sv * kd.fh;
+            T cr[3];
+            cross_vr<D, T>(v, d, cr);
+            Dv_a.add(mj * sv * kd.f);
+            Dvh_a.add(mj * sv * kd.fh);
 #pragma unroll
             for(int k = 0; k < 3; ++k) {
-                Rv[k] += mj * cr[k] * kd.f;
-                Rvh[k] += mj * cr[k] * kd.fh;
+                Rv_a[k].add(mj * cr[k] * kd.f);
+                Rvh_a[k].add(mj * cr[k] * kd.fh);
             }
         }
     }
+    const double rho = rho_a.val(), rhop = rhop_a.val(), rhopp = rhopp_a.val(), Dv = Dv_a.val(), Dvh = Dvh_a.val();
+    const double Rv[3] = {Rv_a[0].val(), Rv_a[1].val(), Rv_a[2].val()};
+    const double Rvh[3] = {Rvh_a[0].val(), Rvh_a[1].val(), Rvh_a[2].val()};
 
     const double gm1 = a.gamma - 1.0;
     const double rho_f = s.f[F_DENS][i], pres = s.f[F_PRES][i], om = s.f[F_GRADH][i];
@@ -317,13 +345,12 @@ template <int D, int K> __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(
 
 // derivative of the seeded sum terms of one pair with respect to r_ij and
 // v_ij; the terms are even in (r_ij, v_ij), so both ends use the same form
-template <int D, int K>
-__device__ __forceinline__ void sum_term(double r, const double d[D], const double v[D], double sv, double m, double h,
-                                         double srho, double srhop, double sdv, const double srv[3], double xb[D],
-                                         double vb[D])
+template <int D, int K, class T>
+__device__ __forceinline__ void sum_term(T r, const T d[D], const T v[D], T sv, T m, T h, T srho, T srhop, T sdv,
+                                         const T srv[3], Acc<T> xb[D], Acc<T> vb[D])
 {
-    const KernelDerivs<double> kd = kernel_derivs<D, K, double>(r, h);
-    double gQd[D], gQv[D], Q = 0.0;
+    const KernelDerivs<T> kd = kernel_derivs<D, K, T>(r, h);
+    T gQd[D], gQv[D], Q = T(0);
     if constexpr(D == 3) {
         // Q = srv . (v x r):  dQ/dr = srv x v,  dQ/dv = r x srv
         gQd[0] = srv[1] * v[2] - srv[2] * v[1];
@@ -340,62 +367,64 @@ __device__ __forceinline__ void sum_term(double r, const double d[D], const doub
         gQv[1] = -srv[2] * d[0];
         Q = gQd[0] * d[0] + gQd[1] * d[1];
     } else {
-        gQd[0] = gQv[0] = 0.0;
+        gQd[0] = gQv[0] = T(0);
     }
-    const double radial = m * (srho * kd.f * r + srhop * kd.whr + (sdv * sv + Q) * kd.fr) / r;
-    const double mf = m * kd.f;
+    const T radial = m * (srho * kd.f * r + srhop * kd.whr + (sdv * sv + Q) * kd.fr) / r;
+    const T mf = m * kd.f;
 #pragma unroll
     for(int k = 0; k < D; ++k) {
-        xb[k] += radial * d[k] + mf * (sdv * v[k] + gQd[k]);
-        vb[k] += mf * (sdv * d[k] + gQv[k]);
+        xb[k].add(radial * d[k] + mf * (sdv * v[k] + gQd[k]));
+        vb[k].add(mf * (sdv * d[k] + gQv[k]));
     }
 }
 
 // pass 3: adjoint of x and v through the seeded gather sums of i (pairs
 // with r < h_i) and of its neighbours j (pairs with r < h_j)
-template <int D, int K> __global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int n, AdjArgs a)
+template <int D, int K, class T>
+__global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int n, AdjArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
-    double xi[D], vi[D];
+    double xi[D];
+    T vi[D];
 #pragma unroll
     for(int k = 0; k < D; ++k) {
         xi[k] = s.f[F_X + k][i];
-        vi[k] = s.f[F_V + k][i];
+        vi[k] = (T)s.f[F_V + k][i];
     }
-    const double hi = s.f[F_SML][i], mi = s.f[F_MASS][i];
-    const double srho_i = W(a, W_SRHO, i), srhop_i = W(a, W_SRHOP, i), sdv_i = W(a, W_SDV, i);
-    const double srv_i[3] = {W(a, W_SRV, i), W(a, W_SRV + 1, i), W(a, W_SRV + 2, i)};
-    double xb[D], vb[D];
-#pragma unroll
-    for(int k = 0; k < D; ++k) xb[k] = vb[k] = 0.0;
+    const T hi = (T)s.f[F_SML][i], mi = (T)s.f[F_MASS][i];
+    const T srho_i = (T)W(a, W_SRHO, i), srhop_i = (T)W(a, W_SRHOP, i), sdv_i = (T)W(a, W_SDV, i);
+    const T srv_i[3] = {(T)W(a, W_SRV, i), (T)W(a, W_SRV + 1, i), (T)W(a, W_SRV + 2, i)};
+    Acc<T> xb[D], vb[D];
     for(int kk = 0; kk < cnt; ++kk) {
         const int j = a.list[base + (size_t)kk * WAVE];
-        double xj[D], d[D];
+        double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-        const double r = sqrt(sep2<D>(box, xi, xj, d));
-        const double hj = s.f[F_SML][j];
+        const T r = pair_r<T>(sep2<D>(box, xi, xj, dd));
+        const T hj = fld<T>(s, F_SML, j);
         const bool in_i = r < hi, in_j = r < hj;
-        if(r == 0.0 || !(in_i || in_j)) continue;
-        double v[D], sv = 0.0;
+        if(r == T(0) || !(in_i || in_j)) continue;
+        T v[D], d[D], sv = T(0);
 #pragma unroll
         for(int k = 0; k < D; ++k) {
-            v[k] = vi[k] - s.f[F_V + k][j];
+            d[k] = (T)dd[k];
+            v[k] = vi[k] - fld<T>(s, F_V + k, j);
             sv += v[k] * d[k];
         }
-        if(in_i) sum_term<D, K>(r, d, v, sv, s.f[F_MASS][j], hi, srho_i, srhop_i, sdv_i, srv_i, xb, vb);
+        if(in_i) sum_term<D, K, T>(r, d, v, sv, fld<T>(s, F_MASS, j), hi, srho_i, srhop_i, sdv_i, srv_i, xb, vb);
         if(in_j) {
-            const double srv_j[3] = {W(a, W_SRV, j), W(a, W_SRV + 1, j), W(a, W_SRV + 2, j)};
-            sum_term<D, K>(r, d, v, sv, mi, hj, W(a, W_SRHO, j), W(a, W_SRHOP, j), W(a, W_SDV, j), srv_j, xb, vb);
+            const T srv_j[3] = {(T)W(a, W_SRV, j), (T)W(a, W_SRV + 1, j), (T)W(a, W_SRV + 2, j)};
+            sum_term<D, K, T>(r, d, v, sv, mi, hj, (T)W(a, W_SRHO, j), (T)W(a, W_SRHOP, j), (T)W(a, W_SDV, j), srv_j,
+                              xb, vb);
         }
     }
 #pragma unroll
     for(int k = 0; k < D; ++k) {
-        W(a, W_XB + k, i) += xb[k];
-        W(a, W_VB + k, i) += vb[k];
+        W(a, W_XB + k, i) += xb[k].val();
+        W(a, W_VB + k, i) += vb[k].val();
     }
 }
 
@@ -433,14 +462,20 @@ template <int D> __global__ void __launch_bounds__(BLOCK) adj_state_kernel(State
     }
 }
 
-template <int D, int K> static void launch_passes(Ctx &c, const AdjArgs &a, const BoxDev &box)
+template <int D, int K, class T> static void launch_passes_t(Ctx &c, const AdjArgs &a, const BoxDev &box)
 {
     const int g = grid_for(c.n);
     hipLaunchKernelGGL((adj_seed_kernel<D>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, a);
-    hipLaunchKernelGGL((adj_force_kernel<D, K>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
-    hipLaunchKernelGGL((adj_pre_kernel<D, K>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
-    hipLaunchKernelGGL((adj_sum_kernel<D, K>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((adj_force_kernel<D, K, T>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((adj_pre_kernel<D, K, T>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((adj_sum_kernel<D, K, T>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
     hipLaunchKernelGGL((adj_state_kernel<D>), dim3(g), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, a);
+}
+
+template <int D, int K> static void launch_passes(Ctx &c, const AdjArgs &a, const BoxDev &box)
+{
+    if(c.p.precision == SPHGPU_MIXED) launch_passes_t<D, K, float>(c, a, box);
+    else launch_passes_t<D, K, double>(c, a, box);
 }
 
 int adjoint_step(Ctx &c, bool kick)

@@ -569,7 +569,6 @@ static int rollout_supported(const Ctx &c)
     if(p.av_use_time_dependent) return set_error(SPHGPU_E_INVALID, "rollout adjoint: time-dependent viscosity is not differentiated");
     if(p.ac_is_valid) return set_error(SPHGPU_E_INVALID, "rollout adjoint: artificial conductivity is not differentiated");
     if(p.gravity_is_valid) return set_error(SPHGPU_E_INVALID, "rollout adjoint: gravity is not differentiated");
-    if(p.precision != SPHGPU_FP64) return set_error(SPHGPU_E_INVALID, "rollout adjoint: FP64 precision only");
     return SPHGPU_OK;
 }
 
