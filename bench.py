@@ -134,6 +134,17 @@ def adjoint_run(n_side: int, steps: int, dt: float, precision: int) -> dict:
             "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain}, (gx, gv, gu)
 
 
+def workload_rate(name: str, params, fields, steps: int, warmup: int) -> dict:
+    """Throughput of one of the other workloads (mixed precision)."""
+    n = int(len(fields["mass"]))
+    el, stats, timers = timed_run(params, fields, steps, warmup, name)
+    rate = n * steps / el
+    log(f"[bench] {name}: N={n} {rate:.4g} particle-steps/s")
+    return {"n_particles": n, "steps": steps, "particle_steps_per_s": rate, "ms_per_step": el / steps * 1e3,
+            "stage_ms_per_step": {k: v / steps for k, v in timers.items()},
+            "force_pairs_per_s": stats["force_pairs"] / el}
+
+
 def mixed_errors(params_fp64, params_mixed, fields) -> dict:
     """Per-field error of mixed against FP64 after Solver::initialize on the
     benchmark's initial state."""
@@ -149,12 +160,13 @@ def mixed_errors(params_fp64, params_mixed, fields) -> dict:
         res.append(g.download(("dens", "sml", "pres", "acc", "phi")))
         g.close()
     a, b = res
-    rel = lambda x, y: float(np.max(np.abs(x - y) / np.abs(y)))
-    rms = lambda x, y: float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2)))
+    rel = lambda x, y: float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-300)))
+    rms = lambda x, y: float(np.sqrt(np.mean((x - y) ** 2)) / max(np.sqrt(np.mean(y ** 2)), 1e-300))
     out["dens_max_rel"] = rel(b["dens"], a["dens"])
     out["sml_max_rel"] = rel(b["sml"], a["sml"])
     out["pres_max_rel"] = rel(b["pres"], a["pres"])
-    out["phi_max_rel"] = rel(b["phi"], a["phi"])
+    if params_fp64.gravity_is_valid:
+        out["phi_max_rel"] = rel(b["phi"], a["phi"])
     out["acc_rms_rel"] = rms(b["acc"], a["acc"])
     out["bars"] = {"dens_max_rel": 1e-5, "acc_rms_rel": 1e-4}
     return out
@@ -173,6 +185,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout timing")
     ap.add_argument("--adjoint-side", type=int, default=64)
     ap.add_argument("--adjoint-steps", type=int, default=16)
+    ap.add_argument("--adjoint-only", action="store_true", help="time only the differentiable rollout (profiling)")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the KHI and Sedov throughput lines")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -191,6 +205,12 @@ def main() -> int:
         return 2
 
     from sphcode_amd import FP64, MIXED, costs, ics
+
+    if args.adjoint_only:
+        adj, _ = adjoint_run(args.adjoint_side, args.adjoint_steps, 1e-3,
+                             MIXED if args.precision == "mixed" else FP64)
+        print(json.dumps({"adjoint": adj}), flush=True)
+        return 0
 
     n = args.n
     t0 = time.perf_counter()
@@ -241,6 +261,26 @@ def main() -> int:
             name: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for name, a, b in zip(("pos", "vel", "ene"), gmx, g64)}
         log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}")
 
+    workloads = None
+    if not args.no_workloads:
+        workloads = {}
+        pk, fk = ics.khi(precision=MIXED)
+        workloads["khi_disph_2d"] = workload_rate("khi", pk, fk, 5, 1)
+        pk64 = pk.copy()
+        pk64.precision = FP64
+        workloads["khi_disph_2d"]["mixed_vs_fp64"] = mixed_errors(pk64, pk, fk)
+        del fk
+        ps, fs = ics.sedov(n_side=256, precision=MIXED)
+        ps.neighbor_capacity = 256
+        workloads["sedov_gsph_3d"] = workload_rate("sedov", ps, fs, 3, 1)
+        del fs
+        ps2, fs2 = ics.sedov(n_side=128, precision=FP64)
+        ps2.neighbor_capacity = 256
+        pm2 = ps2.copy()
+        pm2.precision = MIXED
+        workloads["sedov_gsph_3d"]["mixed_vs_fp64_at_128^3"] = mixed_errors(ps2, pm2, fs2)
+        del fs2
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p64, fields, n, args.cpu_steps)
@@ -269,6 +309,7 @@ def main() -> int:
         "stage_ms_per_step": {k: v / args.steps for k, v in timers.items()},
         "roofline": roofline,
         "adjoint": adjoint,
+        "other_workloads": workloads,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -156,6 +156,9 @@ typedef struct sphgpu_stats {
     int64_t gravity_opened;       /* cells opened by the gravity walk         */
     int64_t search_nodes;         /* cells visited by neighbour-search walks  */
     int64_t search_candidates;    /* particles tested by neighbour searches   */
+    int64_t gravity_wave_steps;   /* iterations of the gravity walk, summed over
+                                     waves (each visits one node per iteration
+                                     for the lanes whose traversal is there)   */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

@@ -36,6 +36,10 @@ FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are disabled throughout
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division and square root (mixed-precision pair terms only) use the
+    # hardware reciprocal / square root (about 1 ulp) instead of the
+    # correctly rounded sequences; FP64 arithmetic is unaffected
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I", str(ROOT / "include"),
 ]

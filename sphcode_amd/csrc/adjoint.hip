@@ -67,6 +67,7 @@ static int ck_fields(int D, int *f)
 struct AdjArgs {
     const int *list;
     const int *cnt;
+    const int *cnt2;
     int cap;
     double *w;       // work arrays
     int64_t n;       // their stride
@@ -121,6 +122,7 @@ __global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, i
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
+    const int total = cnt + a.cnt2[i];
     double xi[D];
     T vi[D], afi[D];
 #pragma unroll
@@ -136,8 +138,8 @@ __global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, i
     const T Ai = pi_ * omi / (rhoi * rhoi);
 
     Acc<T> xb[D], vb[D], hb, rhob, Ab, Bb, cb;
-    for(int kk = 0; kk < cnt; ++kk) {
-        const int j = a.list[base + (size_t)kk * WAVE];
+    for(int kk = 0; kk < total; ++kk) {
+        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
@@ -239,6 +241,7 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
+    const int total = cnt + a.cnt2[i];
     double xi[D];
     T vi[D];
 #pragma unroll
@@ -248,41 +251,47 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
     }
     const double h = s.f[F_SML][i];
     const T ht = (T)h;
-    Acc<T> rho_a, rhop_a, rhopp_a, Dv_a, Dvh_a, Rv_a[3], Rvh_a[3];
-    for(int kk = 0; kk < cnt; ++kk) {
-        const int j = a.list[base + (size_t)kk * WAVE];
+    Acc<T> rho_a, rhop_a, rhopp_a;
+    // the velocity-derivative sums are FP64 in both precisions, as in the
+    // forward pre-interaction
+    double vi64[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) vi64[k] = s.f[F_V + k][i];
+    double Dv = 0.0, Dvh = 0.0, Rv[3] = {0.0, 0.0, 0.0}, Rvh[3] = {0.0, 0.0, 0.0};
+    for(int kk = 0; kk < total; ++kk) {
+        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-        const T r = pair_r<T>(sep2<D>(box, xi, xj, dd));
+        const double r2 = sep2<D>(box, xi, xj, dd);
+        const T r = pair_r<T>(r2);
         if(!(r < ht)) continue;
         const T mj = fld<T>(s, F_MASS, j);
         const KernelDerivs<T> kd = kernel_derivs<D, K, T>(r, ht);
         rho_a.add(mj * kd.w);
         rhop_a.add(mj * kd.wh);
         rhopp_a.add(mj * kd.whh);
-        if(j != i) {
-            T v[D], d[D], sv = T(0);
+        if(j != i && a.balsara && D > 1) {
+            const KernelDerivs<double> k64 = kernel_derivs<D, K, double>(sqrt(r2), h);
+            const double m64 = s.f[F_MASS][j];
+            double v[D], sv = 0.0;
 #pragma unroll
             for(int k = 0; k < D; ++k) {
-                d[k] = (T)dd[k];
-                v[k] = vi[k] - fld<T>(s, F_V + k, j);
-                sv += v[k] * d[k];
+                v[k] = vi64[k] - s.f[F_V + k][j];
+                sv += v[k] * dd[k];
             }
-            T cr[3];
-            cross_vr<D, T>(v, d, cr);
-            Dv_a.add(mj * sv * kd.f);
-            Dvh_a.add(mj * sv * kd.fh);
+            double cr[3];
+            cross_vr<D, double>(v, dd, cr);
+            Dv += m64 * sv * k64.f;
+            Dvh += m64 * sv * k64.fh;
 #pragma unroll
             for(int k = 0; k < 3; ++k) {
-                Rv_a[k].add(mj * cr[k] * kd.f);
-                Rvh_a[k].add(mj * cr[k] * kd.fh);
+                Rv[k] += m64 * cr[k] * k64.f;
+                Rvh[k] += m64 * cr[k] * k64.fh;
             }
         }
     }
-    const double rho = rho_a.val(), rhop = rhop_a.val(), rhopp = rhopp_a.val(), Dv = Dv_a.val(), Dvh = Dvh_a.val();
-    const double Rv[3] = {Rv_a[0].val(), Rv_a[1].val(), Rv_a[2].val()};
-    const double Rvh[3] = {Rvh_a[0].val(), Rvh_a[1].val(), Rvh_a[2].val()};
+    const double rho = rho_a.val(), rhop = rhop_a.val(), rhopp = rhopp_a.val();
 
     const double gm1 = a.gamma - 1.0;
     const double rho_f = s.f[F_DENS][i], pres = s.f[F_PRES][i], om = s.f[F_GRADH][i];
@@ -387,6 +396,7 @@ __global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int
     if(i >= n) return;
     const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
+    const int total = cnt + a.cnt2[i];
     double xi[D];
     T vi[D];
 #pragma unroll
@@ -398,8 +408,8 @@ __global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int
     const T srho_i = (T)W(a, W_SRHO, i), srhop_i = (T)W(a, W_SRHOP, i), sdv_i = (T)W(a, W_SDV, i);
     const T srv_i[3] = {(T)W(a, W_SRV, i), (T)W(a, W_SRV + 1, i), (T)W(a, W_SRV + 2, i)};
     Acc<T> xb[D], vb[D];
-    for(int kk = 0; kk < cnt; ++kk) {
-        const int j = a.list[base + (size_t)kk * WAVE];
+    for(int kk = 0; kk < total; ++kk) {
+        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
@@ -483,6 +493,7 @@ int adjoint_step(Ctx &c, bool kick)
     AdjArgs a{};
     a.list = c.list;
     a.cnt = c.list_cnt;
+    a.cnt2 = c.list_cnt2;
     a.cap = c.cap;
     a.w = c.ro.work;
     a.n = c.ro.n;

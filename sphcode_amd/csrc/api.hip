@@ -76,7 +76,7 @@ static void free_buffers(Ctx &c)
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
     fr(c.parent); fr(c.nchild); fr(c.arrive); fr(c.msum); fr(c.groups); fr(c.num_groups);
-    fr(c.list); fr(c.list_r); fr(c.list_cnt);
+    fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
     fr(c.rec_gather); fr(c.rec_gather_cnt); fr(c.rec_force); fr(c.rec_force_cnt);
     fr(c.partial);
     c.cap_n = 0;
@@ -103,6 +103,7 @@ static int alloc_buffers(Ctx &c, int64_t n)
     if(mixed) SPHGPU_HIP(hipMalloc(&c.list_rf, lsz * sizeof(float)));
     else SPHGPU_HIP(hipMalloc(&c.list_r, lsz * sizeof(double)));
     SPHGPU_HIP(hipMalloc(&c.list_cnt, np * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.list_cnt2, np * sizeof(int)));
     const int64_t npart = std::max<int64_t>(grid_for(np), 1024) * 8 + 64;
     SPHGPU_HIP(hipMalloc(&c.partial, npart * sizeof(double)));
     int rc = tree_alloc(c);
@@ -136,19 +137,29 @@ static int stage_tree(Ctx &c)
     if(rc) return rc;
     c.tree_ready = true;
     c.have_gather = c.have_force = false;
+    c.lists_symmetric = false;  // lists refer to the previous order
     return SPHGPU_OK;
 }
 
+// The pre-interaction's candidate search.  After the first call, one walk
+// finds both the gather candidates and the symmetric pairs the force stage
+// needs (search_combined); the first call keeps the reference's initial
+// smoothing and separate searches.
 static int stage_pre(Ctx &c)
 {
     int rc;
+    const bool combined = !c.first_pre;
     if(c.first_pre) {
         // PreInteraction::initial_smoothing (src/pre_interaction.cpp:41-44, 171-215)
         if((rc = search_gather(c, true))) return rc;
         if((rc = pre_initial(c))) return rc;
         c.first_pre = false;
+        if((rc = search_gather(c, false))) return rc;
+    } else {
+        if((rc = search_guess(c))) return rc;
+        if((rc = tree_set_kernel(c))) return rc;  // largest search radius per cell
+        if((rc = search_combined(c))) return rc;
     }
-    if((rc = search_gather(c, false))) return rc;
     if(c.record) {
         const size_t lsz = (size_t)c.cap_n * (size_t)c.cap;
         SPHGPU_HIP(hipMemcpyAsync(c.rec_gather, c.list, lsz * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
@@ -156,13 +167,25 @@ static int stage_pre(Ctx &c)
         c.have_gather = true;
     }
     if((rc = pre_compute(c))) return rc;
+    c.lists_symmetric = false;
+    if(combined) {
+        // the combined lists hold every force pair unless some h outgrew its
+        // search radius (src/pre_interaction.cpp:70 truncates the gather set)
+        SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+        SPHGPU_HIP(hipStreamSynchronize(c.stream));
+        const int64_t tr = c.counters_host[C_TRUNC];
+        c.lists_symmetric = tr == c.trunc_seen && c.counters_host[C_OVERFLOW] == 0;
+        c.trunc_seen = tr;
+    }
+    if(c.lists_symmetric) return SPHGPU_OK;
     return tree_set_kernel(c);  // BHTree::set_kernel (src/pre_interaction.cpp:166-168)
 }
 
 static int stage_force(Ctx &c)
 {
     int rc;
-    if((rc = search_symmetric(c))) return rc;
+    if(!c.lists_symmetric && (rc = search_symmetric(c))) return rc;
+    c.lists_symmetric = false;
     if((rc = force_compute(c))) return rc;
     if(c.record) c.have_force = true;
     return SPHGPU_OK;
@@ -358,6 +381,7 @@ int sphgpu_upload(void *ptr, const sphgpu_particles *h)
     }
     // list slots beyond n stay empty
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c.cap_n)), dim3(BLOCK), 0, c.stream, c.list_cnt, c.cap_n, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c.cap_n)), dim3(BLOCK), 0, c.stream, c.list_cnt2, c.cap_n, 0);
     SPHGPU_HIP(hipGetLastError());
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     c.uploaded = true;
@@ -670,6 +694,7 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->gravity_opened = k[C_GRAV_OPEN];
     st->search_nodes = k[C_SEARCH_NODES];
     st->search_candidates = k[C_SEARCH_CAND];
+    st->gravity_wave_steps = k[C_GRAV_STEPS];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];
@@ -706,6 +731,7 @@ int sphgpu_get_timers(void *ptr, sphgpu_timers *t)
 int sphgpu_reset_stats(void *ptr)
 {
     SPHGPU_CTX(ptr);
+    c.trunc_seen = 0;
     hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.counters, NCOUNTERS);
     SPHGPU_HIP(hipGetLastError());
     SPHGPU_HIP(hipStreamSynchronize(c.stream));

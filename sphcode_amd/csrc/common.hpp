@@ -161,6 +161,14 @@ template <int D> __device__ __forceinline__ void wrap(const BoxDev &b, double x[
 // hoisted out of the pair loop.
 
 // Cubic spline with support h: H = h/2, q = r/H < 2.
+// 1/sqrt(x) in the pair arithmetic type (HIP's rsqrt takes double only, so
+// a float argument would be widened and refined in FP64)
+template <class T> __device__ __forceinline__ T rsqrt_t(T x)
+{
+    if constexpr(sizeof(T) == 4) return rsqrtf(x);
+    else return rsqrt(x);
+}
+
 template <int D, class T> struct Cubic {
     static constexpr double sigma_d = D == 1 ? 2.0 / 3.0 : D == 2 ? 10.0 / (7.0 * 3.14159265358979323846) : 1.0 / 3.14159265358979323846;
     T qscale;   // 1/H

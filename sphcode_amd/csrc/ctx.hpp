@@ -61,10 +61,14 @@ struct TreeDev {
     int *num_nodes;  // device scalar
 };
 
+// Slot of entry kk of a list with `front` entries at the start and the rest
+// at the end (see Ctx::list).
+__device__ __forceinline__ int list_slot(int kk, int front, int cap) { return kk < front ? kk : cap - 1 - (kk - front); }
+
 // Device-side counters (int64), reduced per wave before the atomic.
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
-    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, NCOUNTERS
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, NCOUNTERS
 };
 
 // Device scalars.
@@ -104,11 +108,18 @@ struct Ctx {
     int *num_groups = nullptr;   // device scalar
     int64_t group_cap = 0;
 
-    // neighbour lists: list[(wave * cap + k) * 64 + lane]
+    // neighbour lists: list[(wave * cap + k) * 64 + lane].  The first
+    // list_cnt[i] entries (k = 0, 1, ...) are the gather candidates
+    // (r < h_guess * ratio) with |r_ij| in list_r / list_rf; the combined
+    // search appends the remaining symmetric candidates from the other end
+    // (k = cap-1, cap-2, ..., list_cnt2[i] of them).
     int *list = nullptr;
     double *list_r = nullptr;  // |r_ij| of gather candidates (FP64 mode)
     float *list_rf = nullptr;  // |r_ij| of gather candidates (mixed mode)
     int *list_cnt = nullptr;
+    int *list_cnt2 = nullptr;
+    bool lists_symmetric = false;  // the lists hold every force pair
+    int64_t trunc_seen = 0;        // truncation counter at the last check
     int *rec_gather = nullptr, *rec_gather_cnt = nullptr;  // recorded sets (debug)
     int *rec_force = nullptr, *rec_force_cnt = nullptr;
     bool record = false;
@@ -168,6 +179,8 @@ int tree_set_kernel(Ctx &c);     // BHTree::set_kernel
 // search.hip: candidate lists from the tree walk
 int search_gather(Ctx &c, bool initial);  // h from the density guess
 int search_symmetric(Ctx &c);
+int search_guess(Ctx &c);        // sml = h_guess * ratio
+int search_combined(Ctx &c);     // gather and symmetric candidates in one walk
 // pre.hip
 int pre_initial(Ctx &c);         // initial_smoothing density
 int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV switches / MUSCL gradients

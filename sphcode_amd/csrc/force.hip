@@ -15,6 +15,7 @@ namespace sphgpu {
 struct ForceArgs {
     const int *list;
     const int *cnt;
+    const int *cnt2;
     int cap;
     double gamma;
     int use_ac;
@@ -72,6 +73,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
     if(valid) {
         const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
         const int cnt = a.cnt[i];
+        const int total = cnt + a.cnt2[i];
         double xi[D];
         T vi[D];
 #pragma unroll
@@ -99,8 +101,8 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 
         Acc<T> acc[D], dene;
         int used = 0;
-        for(int k = 0; k < cnt; ++k) {
-            const int j = a.list[base + (size_t)k * WAVE];
+        for(int kk = 0; kk < total; ++kk) {
+            const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
             double xj[D], d[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
@@ -281,6 +283,7 @@ int force_compute(Ctx &c)
     ForceArgs a{};
     a.list = c.list;
     a.cnt = c.list_cnt;
+    a.cnt2 = c.list_cnt2;
     a.cap = c.cap;
     a.gamma = c.p.gamma;
     a.use_ac = c.p.ac_is_valid;

@@ -33,7 +33,7 @@ template <int D, class T> struct GravityVisitor {
     Acc<T> phi;
     Acc<T> acc[D];
     double d[D], d2 = 0.0;
-    int64_t npp = 0, npn = 0, nopen = 0;
+    int npp = 0, npn = 0, nopen = 0;  // per lane, well below 2^31
 
     __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_) {}
 
@@ -57,7 +57,7 @@ template <int D, class T> struct GravityVisitor {
     __device__ __forceinline__ void far(int node)
     {
         const T gm = (T)(G * t.mom[node].w);
-        const T rinv = rsqrt((T)d2);
+        const T rinv = rsqrt_t<T>((T)d2);
         phi.add(-(gm * rinv));
         const T c = gm * rinv * rinv * rinv;
 #pragma unroll
@@ -84,7 +84,7 @@ template <int D, class T> struct GravityVisitor {
                     r2 += rij[k] * rij[k];
                 }
                 if constexpr(sizeof(T) == 8) r2 = r2d;
-                const T rinv = r2 > T(0) ? rsqrt(r2) : T(0);
+                const T rinv = r2 > T(0) ? rsqrt_t<T>(r2) : T(0);
                 const T rr = r2 * rinv;
                 // far field of both softening lengths unless one is within 2e
                 T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
@@ -126,17 +126,18 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kerne
     __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
-    tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+    const int visits = tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
         for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k].val();
     }
-    const int64_t pp = wave_sum(v.npp), pn = wave_sum(v.npn), po = wave_sum(v.nopen);
+    const int64_t pp = wave_sum((int64_t)v.npp), pn = wave_sum((int64_t)v.npn), po = wave_sum((int64_t)v.nopen);
     if(wave_lane() == 0) {
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)pp);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)pn);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)po);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)visits);
     }
 }
 

@@ -20,6 +20,12 @@
 // distance (src/bhtree.cpp:120-124) and stops at the first r >= h, whereas
 // the pair loops here test r < h on the unsorted list -- the same set.
 //
+// The combined search (split mode) runs the symmetric test with sml = the
+// gather radius and cells' largest gather radius, and files each pair with
+// r^2 < sml_i^2 (a gather candidate) at the front of the list and the rest
+// at the back, so that one walk serves both the pre-interaction and the
+// force stage.
+//
 // List layout: entry k of the particle in slot (wave w, lane l) lives at
 // (w * cap + k) * 64 + l, so a wave's k-th loads are one contiguous line.
 #include "ctx.hpp"
@@ -28,9 +34,12 @@ namespace sphgpu {
 
 struct SearchArgs {
     int *list;
-    double *list_r;      // |r_ij| of each candidate (gather search, FP64 mode)
-    float *list_rf;      // the same in FP32 (gather search, mixed mode)
-    int *cnt;
+    double *list_r;      // |r_ij| of each gather candidate (FP64 mode)
+    float *list_rf;      // the same in FP32 (mixed mode)
+    int *cnt;            // entries at the front of the list
+    int *cnt2;           // entries at the back of the list
+    int split;           // symmetric search that also marks the gather set:
+                         // r < sml_i at the front, the rest at the back
     int cap;
     int guess;           // 1: set sml = h_guess * ratio first
     double ratio;
@@ -130,7 +139,8 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     int *out = a.list + base;
     double *outr = a.list_r ? a.list_r + base : nullptr;
     float *outrf = a.list_rf ? a.list_rf + base : nullptr;
-    int count = 0;
+    int count = 0, count2 = 0;  // candidates found (front / back)
+    int wf = 0, wb = 0;          // of those, written (at most cap in total)
 
     const int total = uniform(*t.num_nodes);
     int node = 0;
@@ -264,12 +274,22 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                         lim = Hc * Hc;
                     }
                     if(r2 < lim) {
-                        if(count < a.cap) {
-                            out[(size_t)count * WAVE] = c_j[w][c];
-                            if(outr) outr[(size_t)count * WAVE] = sqrt(r2);
-                            if(outrf) outrf[(size_t)count * WAVE] = (float)sqrt(r2);
+                        const bool room = wf + wb < a.cap;
+                        if(!a.split || r2 < h2_own) {
+                            if(room) {
+                                out[(size_t)wf * WAVE] = c_j[w][c];
+                                if(outr) outr[(size_t)wf * WAVE] = sqrt(r2);
+                                if(outrf) outrf[(size_t)wf * WAVE] = (float)sqrt(r2);
+                                ++wf;
+                            }
+                            ++count;
+                        } else {
+                            if(room) {
+                                out[(size_t)(a.cap - 1 - wb) * WAVE] = c_j[w][c];
+                                ++wb;
+                            }
+                            ++count2;
                         }
-                        ++count;
                     }
                 }
             }
@@ -279,9 +299,12 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
 
     int over = 0, len = 0;
     if(valid) {
-        a.cnt[i] = count < a.cap ? count : a.cap;
-        over = count > a.cap ? 1 : 0;
-        len = count;
+        // past the capacity the first cap entries found are kept and the
+        // overflow is reported
+        a.cnt[i] = wf;
+        a.cnt2[i] = wb;
+        over = count + count2 > a.cap ? 1 : 0;
+        len = count + count2;
     }
     over = wave_sum(over);
     len = wave_maxv(len);
@@ -312,6 +335,48 @@ template <bool IJ> static int search_dispatch(Ctx &c, const SearchArgs &a)
     return SPHGPU_OK;
 }
 
+// sml = h_guess * ratio (src/pre_interaction.cpp:61-64), the gather
+// search radius, before the combined search
+template <int D> __global__ void __launch_bounds__(BLOCK) guess_kernel(State s, int n, double nngb, double ratio)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    s.f[F_SML][i] = root_dim<D>(nngb * s.f[F_MASS][i] / (s.f[F_DENS][i] * UnitVolume<D>::value)) * ratio;
+}
+
+int search_guess(Ctx &c)
+{
+    const double ratio = c.p.iterative_sml ? 1.2 : 1.0;
+    const double nngb = (double)c.p.neighbor_number;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL(guess_kernel<1>, dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, nngb, ratio); break;
+    case 2: hipLaunchKernelGGL(guess_kernel<2>, dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, nngb, ratio); break;
+    default: hipLaunchKernelGGL(guess_kernel<3>, dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, nngb, ratio); break;
+    }
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+// One walk for both sets: with sml = h_guess * ratio and the cells' largest
+// sml from tree_set_kernel, the symmetric test r < max(sml_i, ks) finds every
+// gather candidate (r < sml_i, kept at the front with |r_ij|) and every pair
+// with r < max(h_i, h_j) for the final h, as long as no final h exceeds its
+// search radius (the caller checks the truncation counter).
+int search_combined(Ctx &c)
+{
+    SearchArgs a{};
+    a.list = c.list;
+    a.list_r = c.list_r;
+    a.list_rf = c.list_rf;
+    a.cnt = c.list_cnt;
+    a.cnt2 = c.list_cnt2;
+    a.split = 1;
+    a.cap = c.cap;
+    a.guess = 0;
+    a.counters = c.counters;
+    return search_dispatch<true>(c, a);
+}
+
 int search_gather(Ctx &c, bool initial)
 {
     SearchArgs a{};
@@ -319,6 +384,7 @@ int search_gather(Ctx &c, bool initial)
     a.list_r = c.list_r;
     a.list_rf = c.list_rf;
     a.cnt = c.list_cnt;
+    a.cnt2 = c.list_cnt2;
     a.cap = c.cap;
     a.guess = 1;
     a.ratio = initial ? 1.0 : (c.p.iterative_sml ? 1.2 : 1.0);  // src/pre_interaction.cpp:31-35
@@ -334,6 +400,7 @@ int search_symmetric(Ctx &c)
     a.list_r = nullptr;
     a.list_rf = nullptr;
     a.cnt = c.list_cnt;
+    a.cnt2 = c.list_cnt2;
     a.cap = c.cap;
     a.guess = 0;
     a.counters = c.counters;

@@ -27,9 +27,11 @@ constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 //   bool open(int node)                  lane test (active lanes only)
 //   void far(int node)                   node accepted without opening
 //   void leaf(int4 rng, bool open)       whole wave, uniform particle loop
+// Returns the number of node visits of the wave (wave-uniform).
 template <class V>
-__device__ __forceinline__ void tree_walk(const TreeDev &t, int total, bool valid, V &v, int *stk_node, uint64_t *stk_mask)
+__device__ __forceinline__ int tree_walk(const TreeDev &t, int total, bool valid, V &v, int *stk_node, uint64_t *stk_mask)
 {
+    int visits = 0;
     const int lane = wave_lane();
     uint64_t active = __ballot(valid);
     int node = 0;
@@ -64,6 +66,7 @@ __device__ __forceinline__ void tree_walk(const TreeDev &t, int total, bool vali
             continue;
         }
         node = uniform(node);
+        ++visits;
         const int4 r = t.rng[node];
         const bool leaf = (r.w & 256) != 0;
         const int skip = r.z > node ? r.z : node + 1;
@@ -99,6 +102,7 @@ __device__ __forceinline__ void tree_walk(const TreeDev &t, int total, bool vali
             node = skip;
         }
     }
+    return visits;
 }
 
 }  // namespace sphgpu
