@@ -249,7 +249,10 @@ def main() -> int:
     roofline = {"bound": "valu", "kernel": "gravity_kernel<3>", "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                 "flops_per_launch": flops, "launch_ms": grav_s * 1e3,
-                "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s}
+                "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s,
+                # node visits of the lanes over 64 x the waves' walk iterations
+                "walk_lane_utilisation": (stats["gravity_opened"] + stats["gravity_pn"]) /
+                                         max(64 * stats["gravity_wave_steps"], 1)}
 
     adjoint = None
     if not args.no_adjoint:
@@ -259,27 +262,46 @@ def main() -> int:
         adjoint["fp64"] = adj64
         adjoint["grad_rel_err_vs_fp64"] = {
             name: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for name, a, b in zip(("pos", "vel", "ene"), gmx, g64)}
-        log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}")
+        # The difference sits on a few particles at the non-smooth points of
+        # the viscosity and Balsara switch, where the two trajectories fall on
+        # different sides; the rest of the field agrees to round-off.
+        rest = {}
+        for name, a, b in zip(("pos", "vel", "ene"), gmx, g64):
+            e = np.sqrt(((a - b).reshape(len(b), -1) ** 2).sum(axis=1))
+            keep = np.argsort(e)[:-100]
+            rest[name] = float(np.sqrt((e[keep] ** 2).sum()) / np.linalg.norm(b))
+        adjoint["grad_rel_err_vs_fp64_without_100_largest"] = rest
+        log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}, without the 100 "
+            f"largest differences: {rest}")
 
     workloads = None
     if not args.no_workloads:
         workloads = {}
-        pk, fk = ics.khi(precision=MIXED)
-        workloads["khi_disph_2d"] = workload_rate("khi", pk, fk, 5, 1)
-        pk64 = pk.copy()
-        pk64.precision = FP64
-        workloads["khi_disph_2d"]["mixed_vs_fp64"] = mixed_errors(pk64, pk, fk)
-        del fk
-        ps, fs = ics.sedov(n_side=256, precision=MIXED)
-        ps.neighbor_capacity = 256
-        workloads["sedov_gsph_3d"] = workload_rate("sedov", ps, fs, 3, 1)
-        del fs
-        ps2, fs2 = ics.sedov(n_side=128, precision=FP64)
-        ps2.neighbor_capacity = 256
-        pm2 = ps2.copy()
-        pm2.precision = MIXED
-        workloads["sedov_gsph_3d"]["mixed_vs_fp64_at_128^3"] = mixed_errors(ps2, pm2, fs2)
-        del fs2
+        try:
+            pk, fk = ics.khi(precision=MIXED)
+            workloads["khi_disph_2d"] = workload_rate("khi", pk, fk, 5, 1)
+            pk64 = pk.copy()
+            pk64.precision = FP64
+            workloads["khi_disph_2d"]["mixed_vs_fp64"] = mixed_errors(pk64, pk, fk)
+            del fk
+        except Exception as e:  # a side measurement must not lose the main line
+            workloads["khi_disph_2d"] = {"error": str(e)}
+        try:
+            # Only the first step is timed: with u_bg = 1e-5 the reference
+            # scheme leaves non-finite states after it (the MUSCL-reconstructed
+            # pressure of the blast front goes negative), as the oracle does.
+            ps, fs = ics.sedov(n_side=256, precision=MIXED)
+            ps.neighbor_capacity = 256
+            workloads["sedov_gsph_3d"] = workload_rate("sedov", ps, fs, 1, 0)
+            del fs
+            ps2, fs2 = ics.sedov(n_side=128, precision=FP64)
+            ps2.neighbor_capacity = 256
+            pm2 = ps2.copy()
+            pm2.precision = MIXED
+            workloads["sedov_gsph_3d"]["mixed_vs_fp64_at_128^3"] = mixed_errors(ps2, pm2, fs2)
+            del fs2
+        except Exception as e:
+            workloads["sedov_gsph_3d"] = {"error": str(e)}
 
     cpu = None
     if not args.no_cpu:

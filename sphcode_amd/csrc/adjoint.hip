@@ -251,13 +251,7 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
     }
     const double h = s.f[F_SML][i];
     const T ht = (T)h;
-    Acc<T> rho_a, rhop_a, rhopp_a;
-    // the velocity-derivative sums are FP64 in both precisions, as in the
-    // forward pre-interaction
-    double vi64[D];
-#pragma unroll
-    for(int k = 0; k < D; ++k) vi64[k] = s.f[F_V + k][i];
-    double Dv = 0.0, Dvh = 0.0, Rv[3] = {0.0, 0.0, 0.0}, Rvh[3] = {0.0, 0.0, 0.0};
+    Acc<T> rho_a, rhop_a, rhopp_a, Dv_a, Dvh_a, Rv_a[3], Rvh_a[3];
     for(int kk = 0; kk < total; ++kk) {
         const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
         double xj[D], dd[D];
@@ -272,26 +266,27 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
         rhop_a.add(mj * kd.wh);
         rhopp_a.add(mj * kd.whh);
         if(j != i && a.balsara && D > 1) {
-            const KernelDerivs<double> k64 = kernel_derivs<D, K, double>(sqrt(r2), h);
-            const double m64 = s.f[F_MASS][j];
-            double v[D], sv = 0.0;
+            T v[D], d[D], sv = T(0);
 #pragma unroll
             for(int k = 0; k < D; ++k) {
-                v[k] = vi64[k] - s.f[F_V + k][j];
-                sv += v[k] * dd[k];
+                d[k] = (T)dd[k];
+                v[k] = vi[k] - fld<T>(s, F_V + k, j);
+                sv += v[k] * d[k];
             }
-            double cr[3];
-            cross_vr<D, double>(v, dd, cr);
-            Dv += m64 * sv * k64.f;
-            Dvh += m64 * sv * k64.fh;
+            T cr[3];
+            cross_vr<D, T>(v, d, cr);
+            Dv_a.add(mj * sv * kd.f);
+            Dvh_a.add(mj * sv * kd.fh);
 #pragma unroll
             for(int k = 0; k < 3; ++k) {
-                Rv[k] += m64 * cr[k] * k64.f;
-                Rvh[k] += m64 * cr[k] * k64.fh;
+                Rv_a[k].add(mj * cr[k] * kd.f);
+                Rvh_a[k].add(mj * cr[k] * kd.fh);
             }
         }
     }
-    const double rho = rho_a.val(), rhop = rhop_a.val(), rhopp = rhopp_a.val();
+    const double rho = rho_a.val(), rhop = rhop_a.val(), rhopp = rhopp_a.val(), Dv = Dv_a.val(), Dvh = Dvh_a.val();
+    const double Rv[3] = {Rv_a[0].val(), Rv_a[1].val(), Rv_a[2].val()};
+    const double Rvh[3] = {Rvh_a[0].val(), Rvh_a[1].val(), Rvh_a[2].val()};
 
     const double gm1 = a.gamma - 1.0;
     const double rho_f = s.f[F_DENS][i], pres = s.f[F_PRES][i], om = s.f[F_GRADH][i];

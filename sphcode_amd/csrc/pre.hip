@@ -161,22 +161,12 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const double ci = s.f[F_SOUND][i];
         const T ht = (T)h;
         const Kern ker(ht);
-        Acc<T> rho, drho, nd, dnd, pb, dpb;
+        Acc<T> rho, drho, nd, dnd, pb, dpb, divv;
         T vsig_max = (T)(ci * 2.0);
+        CurlAcc<D, T> rot;
         Acc<T> gd[D], gu[D], gv[D][D];
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
         const T ci_t = (T)ci, ui_t = (T)ui;
-        // The velocity-derivative sums of the Balsara switch and the
-        // time-dependent viscosity are FP64 in both precisions: where the
-        // flow is nearly uniform, div v and curl v are small residuals of
-        // their terms, and the switch (and its adjoint) divides by them.
-        using Kern64 = typename KernelOf<D, K, double>::type;
-        const Kern64 ker64(h);
-        Acc<double> divv;
-        CurlAcc<D, double> rot;
-        double vi64[D];
-#pragma unroll
-        for(int k = 0; k < D; ++k) vi64[k] = s.f[F_V + k][i];
         int count = 0;
         for(int k = 0; k < cnt; ++k) {
             const T r = list_r<T>(a, base + (size_t)k * WAVE);
@@ -201,12 +191,11 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 rho.add(mj * w);
             }
             T rij[D], vij[D];
-            double d[D], r2d = 0.0;
             if(j != i || need_dv) {
-                double xj[D];
+                double xj[D], d[D];
 #pragma unroll
                 for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
-                r2d = sep2<D>(box, xi, xj, d);
+                sep2<D>(box, xi, xj, d);
 #pragma unroll
                 for(int l = 0; l < D; ++l) {
                     rij[l] = (T)d[l];
@@ -221,11 +210,11 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 vsig_max = vs > vsig_max ? vs : vsig_max;
             }
             if(need_dv) {
-                if constexpr(TYPE == SPHGPU_GSPH) {
-                    const T cw = ker.dw_coef(q, r);
-                    T dw[D];
+                const T cw = ker.dw_coef(q, r);
+                T dw[D];
 #pragma unroll
-                    for(int l = 0; l < D; ++l) dw[l] = rij[l] * cw;
+                for(int l = 0; l < D; ++l) dw[l] = rij[l] * cw;
+                if constexpr(TYPE == SPHGPU_GSPH) {
                     const T du = mj * (fld<T>(s, F_ENE, j) - ui_t);
 #pragma unroll
                     for(int l = 0; l < D; ++l) {
@@ -239,21 +228,12 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                         for(int l = 0; l < D; ++l) gv[c][l].add(dw[l] * dvc);
                     }
                 } else {
-                    const double r64 = sqrt(r2d);
-                    const double cw = ker64.dw_coef(ker64.q_of(r64), r64);
-                    double dw[D], v64[D];
+                    const T wgt = TYPE == SPHGPU_DISPH ? mj * fld<T>(s, F_ENE, j) : mj;
+                    T vd = vij[0] * dw[0];
 #pragma unroll
-                    for(int l = 0; l < D; ++l) {
-                        dw[l] = d[l] * cw;
-                        v64[l] = vi64[l] - s.f[F_V + l][j];
-                    }
-                    const double m64 = s.f[F_MASS][j];
-                    const double wgt = TYPE == SPHGPU_DISPH ? m64 * s.f[F_ENE][j] : m64;
-                    double vd = v64[0] * dw[0];
-#pragma unroll
-                    for(int l = 1; l < D; ++l) vd += v64[l] * dw[l];
+                    for(int l = 1; l < D; ++l) vd += vij[l] * dw[l];
                     divv.add(-(wgt * vd));
-                    rot.add(v64, dw, wgt);
+                    rot.add(vij, dw, wgt);
                 }
             }
         }

@@ -20,11 +20,11 @@
 // distance (src/bhtree.cpp:120-124) and stops at the first r >= h, whereas
 // the pair loops here test r < h on the unsorted list -- the same set.
 //
-// The combined search (split mode) runs the symmetric test with sml = the
-// gather radius and cells' largest gather radius, and files each pair with
-// r^2 < sml_i^2 (a gather candidate) at the front of the list and the rest
-// at the back, so that one walk serves both the pre-interaction and the
-// force stage.
+// The combined search (split mode) walks with sml = the gather radius and
+// the cells' largest gather radius, keeps pairs with r < max(sml_i, sml_j),
+// and files those with r^2 < sml_i^2 (the gather candidates) at the front of
+// the list and the rest at the back, so that one walk serves both the
+// pre-interaction and the force stage.
 //
 // List layout: entry k of the particle in slot (wave w, lane l) lives at
 // (w * cap + k) * 64 + l, so a wave's k-th loads are one contiguous line.
@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     __shared__ double l_ks[SW][LEAFCAP];
     __shared__ double c_x[SW][3][WAVE];
     __shared__ double c_ks[SW][WAVE];
+    __shared__ double c_h[SW][WAVE];  // the candidate's own radius (split mode)
     __shared__ int c_j[SW][WAVE];
 
     const int w = uniform(threadIdx.x / WAVE);
@@ -256,7 +257,10 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                 c_j[w][pos] = j;
 #pragma unroll
                 for(int k = 0; k < D; ++k) c_x[w][k][pos] = xj[k];
-                if constexpr(IJ) c_ks[w][pos] = ksj;
+                if constexpr(IJ) {
+                    c_ks[w][pos] = ksj;
+                    if(a.split) c_h[w][pos] = s.f[F_SML][j];
+                }
             }
             const int m = uniform(__popcll(kept));
             tested += m;
@@ -269,7 +273,10 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                     const double r2 = sep2<D>(box, xi, xj, d);
                     double lim = h2_own;
                     if constexpr(IJ) {
-                        const double kl = c_ks[w][c];
+                        // split mode: r < max(sml_i, sml_j) (each radius
+                        // bounds that particle's final h unless truncated);
+                        // otherwise the reference's r < max(h_i, leaf's h)
+                        const double kl = a.split ? c_h[w][c] : c_ks[w][c];
                         const double Hc = h < kl ? kl : h;
                         lim = Hc * Hc;
                     }
@@ -358,7 +365,7 @@ int search_guess(Ctx &c)
 }
 
 // One walk for both sets: with sml = h_guess * ratio and the cells' largest
-// sml from tree_set_kernel, the symmetric test r < max(sml_i, ks) finds every
+// sml from tree_set_kernel, the test r < max(sml_i, sml_j) finds every
 // gather candidate (r < sml_i, kept at the front with |r_ij|) and every pair
 // with r < max(h_i, h_j) for the final h, as long as no final h exceeds its
 // search radius (the caller checks the truncation counter).

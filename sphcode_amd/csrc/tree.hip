@@ -69,18 +69,24 @@ template <int D> __global__ void __launch_bounds__(BLOCK) bbox_partial(State s, 
 }
 
 // Root cube: centre (max+min)/2, edge = largest extent (src/bhtree.cpp:80-94).
+// One wave: lanes stride over the block partials, then a wave min/max
+// (exact, so the result does not depend on the order).
 template <int D> __global__ void bbox_final(const double *partial, int nblocks, double *scal)
 {
 #pragma clang fp contract(off)
-    if(threadIdx.x != 0 || blockIdx.x != 0) return;
     double lo[D], hi[D];
-    for(int k = 0; k < D; ++k) { lo[k] = partial[k]; hi[k] = partial[D + k]; }
-    for(int b = 1; b < nblocks; ++b)
+    for(int k = 0; k < D; ++k) { lo[k] = 1.7976931348623157e308; hi[k] = -1.7976931348623157e308; }
+    for(int b = threadIdx.x; b < nblocks; b += WAVE)
         for(int k = 0; k < D; ++k) {
             const double a = partial[b * 2 * D + k], c = partial[b * 2 * D + D + k];
             lo[k] = a < lo[k] ? a : lo[k];
             hi[k] = c > hi[k] ? c : hi[k];
         }
+    for(int k = 0; k < D; ++k) {
+        lo[k] = wave_minv(lo[k]);
+        hi[k] = wave_maxv(hi[k]);
+    }
+    if(threadIdx.x != 0) return;
     double edge = 0.0;
     for(int k = 0; k < D; ++k) {
         scal[S_ROOT_CX + k] = (hi[k] + lo[k]) * 0.5;
