@@ -75,7 +75,9 @@ static void free_buffers(Ctx &c)
     fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
-    fr(c.parent); fr(c.nchild); fr(c.arrive); fr(c.msum); fr(c.groups); fr(c.num_groups);
+    fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
+    if(c.lvl_host) (void)hipHostFree(c.lvl_host);
+    c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
     fr(c.rec_gather); fr(c.rec_gather_cnt); fr(c.rec_force); fr(c.rec_force_cnt);
     fr(c.partial);
@@ -266,6 +268,7 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     c->p = p;
     c->D = p.dim;
     c->cap = p.neighbor_capacity > 0 ? p.neighbor_capacity : p.neighbor_number * 20;
+    c->cap = (c->cap + LIST_C - 1) / LIST_C * LIST_C;  // whole chunks (list_at)
     c->time = p.time_start;
     c->dt = 0.0;
     int dev = 0;
@@ -764,10 +767,9 @@ int sphgpu_neighbor_sets(void *ptr, int kind, int32_t *counts, int32_t *lists, i
     SPHGPU_HIP(hipMemcpy(lst.data(), kind == 0 ? c.rec_gather : c.rec_force, lsz * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<std::vector<int>> sets(n);
     for(int64_t i = 0; i < n; ++i) {
-        const size_t base = (size_t)(i / WAVE) * c.cap * WAVE + i % WAVE;
         std::vector<int> &m = sets[id[i]];
         m.resize(cnt[i]);
-        for(int k = 0; k < cnt[i]; ++k) m[k] = id[lst[base + (size_t)k * WAVE]];
+        for(int k = 0; k < cnt[i]; ++k) m[k] = id[lst[list_at(i, k, c.cap)]];
         std::sort(m.begin(), m.end());
     }
     int64_t off = 0;

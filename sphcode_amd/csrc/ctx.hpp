@@ -61,6 +61,19 @@ struct TreeDev {
     int *num_nodes;  // device scalar
 };
 
+// Neighbour-list addressing.  The lists of 64 consecutive particles are
+// interleaved in chunks of LIST_C entries: entry k of particle i lives at
+//   ((i / 64) * (cap / LIST_C) + k / LIST_C) * 64 * LIST_C + (i % 64) * LIST_C + k % LIST_C.
+// A search lane appending to its list fills whole LIST_C * 4-byte sectors,
+// and a wave reading entry k of its 64 lists touches 64 * LIST_C * 4 bytes
+// that serve the next LIST_C - 1 entries too.  cap is a multiple of LIST_C.
+constexpr int LIST_C = 8;
+__device__ __host__ __forceinline__ size_t list_at(int i, int k, int cap)
+{
+    return ((size_t)(i / WAVE) * (size_t)(cap / LIST_C) + (size_t)(k / LIST_C)) * (WAVE * LIST_C) +
+           (size_t)(i % WAVE) * LIST_C + (size_t)(k % LIST_C);
+}
+
 // Slot of entry kk of a list with `front` entries at the start and the rest
 // at the end (see Ctx::list).
 __device__ __forceinline__ int list_slot(int kk, int front, int cap) { return kk < front ? kk : cap - 1 - (kk - front); }
@@ -100,7 +113,10 @@ struct Ctx {
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     TreeDev t{};
-    int *parent = nullptr, *nchild = nullptr, *arrive = nullptr;
+    int *parent = nullptr, *nchild = nullptr;
+    int *lvl_nodes = nullptr;  // node ids grouped by tree level
+    int *lvl = nullptr;        // per-level node counts, offsets, fill (device)
+    int *lvl_host = nullptr;   // per-level counts and offsets (pinned host)
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
     // search groups: particle ranges of at most 64 particles, each inside
     // the deepest cell holding at most 64 particles
@@ -108,7 +124,7 @@ struct Ctx {
     int *num_groups = nullptr;   // device scalar
     int64_t group_cap = 0;
 
-    // neighbour lists: list[(wave * cap + k) * 64 + lane].  The first
+    // neighbour lists (addressing: list_at).  The first
     // list_cnt[i] entries (k = 0, 1, ...) are the gather candidates
     // (r < h_guess * ratio) with |r_ij| in list_r / list_rf; the combined
     // search appends the remaining symmetric candidates from the other end

@@ -279,15 +279,14 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
     }
 }
 
-// Parent links and child counts; resets arrival counters.
-__global__ void __launch_bounds__(BLOCK) link_kernel(TreeDev t, int *parent, int *nchild, int *arrive)
+// Parent links and child counts.
+__global__ void __launch_bounds__(BLOCK) link_kernel(TreeDev t, int *parent, int *nchild)
 {
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     const int total = *t.num_nodes;
     if(p >= total) return;
     if(p == 0) parent[0] = -1;
     const int4 r = t.rng[p];
-    arrive[p] = 0;
     if(r.w & 256) { nchild[p] = 0; return; }
     int cnt = 0;
     for(int c = p + 1; c < r.z && c < total && cnt < 8; c = t.rng[c].z) {
@@ -303,51 +302,99 @@ __global__ void __launch_bounds__(BLOCK) reset_parent_kernel(int64_t cap, int *p
     if(p < cap) parent[p] = -1;
 }
 
-__global__ void __launch_bounds__(BLOCK) reset_arrive_kernel(const int *num_nodes, int *arrive)
+// ------------------------------------------------- level-by-level reductions
+// Node sums are formed bottom-up one tree level per launch: a launch
+// boundary orders the children's results before the parents read them, so
+// no cross-XCD atomics or cache maintenance are needed.  The nodes of each
+// level are listed once per tree build (level_bins / level_scatter).
+constexpr int NLEVEL = 72;  // > deepest level (max_level + 1 <= 65)
+
+__device__ __forceinline__ int node_level(const int4 &r) { return (r.w & 255) < NLEVEL ? (r.w & 255) : NLEVEL - 1; }
+
+__global__ void fill_level_kernel(int *lvl)
 {
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
-    if(p < *num_nodes) arrive[p] = 0;
+    for(int k = threadIdx.x; k < 3 * NLEVEL; k += blockDim.x) lvl[k] = 0;
 }
 
-__device__ __forceinline__ bool last_arrival(int *arrive, const int *nchild, int q)
+// per-level node counts (block histograms, one atomic per bin and block)
+__global__ void __launch_bounds__(BLOCK) level_count_kernel(TreeDev t, int *lvl)
 {
-    const int old = __hip_atomic_fetch_add(&arrive[q], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    return old == nchild[q] - 1;
-}
-
-// Mass and centre of mass, leaves first then up the tree.
-template <int D>
-__global__ void __launch_bounds__(BLOCK) moments_kernel(State s, TreeDev t, const int *parent, const int *nchild, int *arrive, double4 *msum)
-{
+    __shared__ int bins[NLEVEL];
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK) bins[k] = 0;
+    __syncthreads();
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     const int total = *t.num_nodes;
-    if(p >= total) return;
-    const int4 r = t.rng[p];
-    if(!(r.w & 256)) return;
-    double m = 0.0, mx[3] = {0.0, 0.0, 0.0};
-    for(int i = r.x; i < r.y; ++i) {
-        const double mi = s.f[F_MASS][i];
-        m += mi;
-#pragma unroll
-        for(int k = 0; k < D; ++k) mx[k] += s.f[F_X + k][i] * mi;
+    if(p < total) atomicAdd(&bins[node_level(t.rng[p])], 1);
+    __syncthreads();
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK)
+        if(bins[k]) atomicAdd(&lvl[k], bins[k]);
+}
+
+// offsets (lvl[NLEVEL + L]) from the counts; clears the fill counters
+__global__ void level_offsets_kernel(int *lvl)
+{
+    if(threadIdx.x != 0) return;
+    int o = 0;
+    for(int k = 0; k < NLEVEL; ++k) {
+        lvl[NLEVEL + k] = o;
+        lvl[2 * NLEVEL + k] = 0;
+        o += lvl[k];
     }
-    msum[p] = make_double4(mx[0], mx[1], mx[2], m);
-    t.mom[p] = make_double4(mx[0] / m, mx[1] / m, mx[2] / m, m);
-    int q = parent[p];
-    for(int depth = 0; q >= 0 && q < total && depth < 64; ++depth) {
-        if(!last_arrival(arrive, nchild, q)) return;
-        const int4 rq = t.rng[q];
-        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(BLOCK) level_scatter_kernel(TreeDev t, int *lvl, int *nodes)
+{
+    __shared__ int bins[NLEVEL], base[NLEVEL];
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK) bins[k] = 0;
+    __syncthreads();
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int total = *t.num_nodes;
+    int L = 0, rank = 0;
+    if(p < total) {
+        L = node_level(t.rng[p]);
+        rank = atomicAdd(&bins[L], 1);
+    }
+    __syncthreads();
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK)
+        base[k] = bins[k] ? atomicAdd(&lvl[2 * NLEVEL + k], bins[k]) : 0;
+    __syncthreads();
+    if(p < total) nodes[lvl[NLEVEL + L] + base[L] + rank] = p;
+}
+
+// Mass and centre of mass of the nodes of one level: a leaf sums its
+// particles, an internal node its children (one level deeper, done by the
+// previous launch), in child order.  The root's moments stay zero, as the
+// reference never accumulates them (include/bhtree.hpp:46-53).
+template <int D>
+__global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t, const int *nodes, int count, double4 *msum)
+{
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if(k >= count) return;
+    const int p = nodes[k];
+    const int total = *t.num_nodes;
+    const int4 r = t.rng[p];
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    if(r.w & 256) {
+        double mx[3] = {0.0, 0.0, 0.0};
+        for(int i = r.x; i < r.y; ++i) {
+            const double mi = s.f[F_MASS][i];
+            acc.w += mi;
+#pragma unroll
+            for(int q = 0; q < D; ++q) mx[q] += s.f[F_X + q][i] * mi;
+        }
+        acc.x = mx[0];
+        acc.y = mx[1];
+        acc.z = mx[2];
+    } else {
         int guard = 0;
-        for(int c = q + 1; c < rq.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
+        for(int c = p + 1; c < r.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
             const double4 v = msum[c];
             acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
         }
-        msum[q] = acc;
-        if(q == 0) t.mom[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // root: never accumulated
-        else t.mom[q] = make_double4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
-        q = parent[q];
     }
+    msum[p] = acc;
+    if(p == 0) t.mom[0] = make_double4(0.0, 0.0, 0.0, 0.0);
+    else t.mom[p] = make_double4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
 }
 
 // Largest smoothing length per subtree (BHNode::set_kernel).
@@ -377,32 +424,29 @@ __global__ void __launch_bounds__(BLOCK) groups_kernel(TreeDev t, const int *par
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) kernel_size_kernel(State s, TreeDev t, const int *parent, const int *nchild, int *arrive)
+// Largest smoothing length per subtree (BHNode::set_kernel,
+// src/bhtree.cpp:206-232), one level per launch as the moments.
+__global__ void __launch_bounds__(BLOCK) kernel_size_level_kernel(State s, TreeDev t, const int *nodes, int count)
 {
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if(k >= count) return;
+    const int p = nodes[k];
     const int total = *t.num_nodes;
-    if(p >= total) return;
     const int4 r = t.rng[p];
-    if(!(r.w & 256)) return;
     double h = 0.0;
-    for(int i = r.x; i < r.y; ++i) {
-        const double hi = s.f[F_SML][i];
-        h = hi > h ? hi : h;
+    if(r.w & 256) {
+        for(int i = r.x; i < r.y; ++i) {
+            const double hi = s.f[F_SML][i];
+            h = hi > h ? hi : h;
+        }
+    } else {
+        int guard = 0;
+        for(int c = p + 1; c < r.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
+            const double v = t.ks[c];
+            h = v > h ? v : h;
+        }
     }
     t.ks[p] = h;
-    int q = parent[p];
-    for(int depth = 0; q >= 0 && q < total && depth < 64; ++depth) {
-        if(!last_arrival(arrive, nchild, q)) return;
-        const int4 rq = t.rng[q];
-        double m = 0.0;
-        int guard = 0;
-        for(int c = q + 1; c < rq.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
-            const double v = t.ks[c];
-            m = v > m ? v : m;
-        }
-        t.ks[q] = m;
-        q = parent[q];
-    }
 }
 
 // ------------------------------------------------------------------ host
@@ -424,8 +468,11 @@ int tree_alloc(Ctx &c)
     SPHGPU_HIP(hipMalloc(&c.t.num_nodes, sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.parent, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.nchild, c.node_cap * sizeof(int)));
-    SPHGPU_HIP(hipMalloc(&c.arrive, c.node_cap * sizeof(int)));
+    SPHGPU_HIP(hipMalloc(&c.lvl_nodes, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.msum, c.node_cap * sizeof(double4)));
+    if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
+    if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, 2 * NLEVEL * sizeof(int), hipHostMallocDefault));
+    for(int k = 0; k < 2 * NLEVEL; ++k) c.lvl_host[k] = 0;
     c.group_cap = c.node_cap + n / 64 + 1;
     SPHGPU_HIP(hipMalloc(&c.groups, c.group_cap * sizeof(int2)));
     SPHGPU_HIP(hipMalloc(&c.num_groups, sizeof(int)));
@@ -480,8 +527,20 @@ template <int D> static int tree_build_d(Ctx &c)
                        c.scal, c.t, c.node_cap, c.counters);
     const int ngrid = grid_for(c.node_cap);
     hipLaunchKernelGGL(reset_parent_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.node_cap, c.parent);
-    hipLaunchKernelGGL(link_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.nchild, c.arrive);
-    hipLaunchKernelGGL(moments_kernel<D>, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.s, c.t, c.parent, c.nchild, c.arrive, c.msum);
+    hipLaunchKernelGGL(link_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.nchild);
+    // nodes listed by level (in c.lvl_nodes), counts to the host
+    hipLaunchKernelGGL(fill_level_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.lvl);
+    hipLaunchKernelGGL(level_count_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl);
+    hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.lvl);
+    hipLaunchKernelGGL(level_scatter_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl, c.lvl_nodes);
+    SPHGPU_HIP(hipMemcpyAsync(c.lvl_host, c.lvl, 2 * NLEVEL * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    for(int L = NLEVEL - 1; L >= 0; --L) {
+        const int cnt = c.lvl_host[L];
+        if(cnt <= 0) continue;
+        hipLaunchKernelGGL(moments_level_kernel<D>, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
+                           c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
+    }
     hipLaunchKernelGGL(reset_groups_kernel, dim3(1), dim3(1), 0, c.stream, c.num_groups);
     hipLaunchKernelGGL(groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.groups, c.num_groups, c.group_cap);
     SPHGPU_HIP(hipGetLastError());
@@ -499,9 +558,13 @@ int tree_build(Ctx &c)
 
 int tree_set_kernel(Ctx &c)
 {
-    const int ngrid = grid_for(c.node_cap);
-    hipLaunchKernelGGL(reset_arrive_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t.num_nodes, c.arrive);
-    hipLaunchKernelGGL(kernel_size_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.s, c.t, c.parent, c.nchild, c.arrive);
+    // level lists and counts from the last tree build
+    for(int L = NLEVEL - 1; L >= 0; --L) {
+        const int cnt = c.lvl_host[L];
+        if(cnt <= 0) continue;
+        hipLaunchKernelGGL(kernel_size_level_kernel, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
+                           c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt);
+    }
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
