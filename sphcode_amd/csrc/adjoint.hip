@@ -120,7 +120,6 @@ __global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, i
     constexpr bool FP64 = sizeof(T) == 8;
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
-    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
     const int total = cnt + a.cnt2[i];
     double xi[D];
@@ -139,7 +138,7 @@ __global__ void __launch_bounds__(BLOCK) adj_force_kernel(State s, BoxDev box, i
 
     Acc<T> xb[D], vb[D], hb, rhob, Ab, Bb, cb;
     for(int kk = 0; kk < total; ++kk) {
-        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
+        const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
@@ -239,7 +238,6 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
-    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
     const int total = cnt + a.cnt2[i];
     double xi[D];
@@ -253,7 +251,7 @@ __global__ void __launch_bounds__(BLOCK) adj_pre_kernel(State s, BoxDev box, int
     const T ht = (T)h;
     Acc<T> rho_a, rhop_a, rhopp_a, Dv_a, Dvh_a, Rv_a[3], Rvh_a[3];
     for(int kk = 0; kk < total; ++kk) {
-        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
+        const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
@@ -389,7 +387,6 @@ __global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
-    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
     const int total = cnt + a.cnt2[i];
     double xi[D];
@@ -404,7 +401,7 @@ __global__ void __launch_bounds__(BLOCK) adj_sum_kernel(State s, BoxDev box, int
     const T srv_i[3] = {(T)W(a, W_SRV, i), (T)W(a, W_SRV + 1, i), (T)W(a, W_SRV + 2, i)};
     Acc<T> xb[D], vb[D];
     for(int kk = 0; kk < total; ++kk) {
-        const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
+        const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
         double xj[D], dd[D];
 #pragma unroll
         for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];

@@ -71,7 +71,6 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
     const bool valid = i < n;
     int64_t pairs = 0;
     if(valid) {
-        const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
         const int cnt = a.cnt[i];
         const int total = cnt + a.cnt2[i];
         double xi[D];
@@ -102,7 +101,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         Acc<T> acc[D], dene;
         int used = 0;
         for(int kk = 0; kk < total; ++kk) {
-            const int j = a.list[base + (size_t)list_slot(kk, cnt, a.cap) * WAVE];
+            const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
             double xj[D], d[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
@@ -117,7 +116,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 if(!(r2 < hmax * hmax) || r2 == 0.0) continue;
                 r = sqrtf((float)r2);
             }
-            if(a.rec) a.rec[base + (size_t)used * WAVE] = j;
+            if(a.rec) a.rec[list_at(i, used, a.cap)] = j;
             ++used;
             T rij[D];
 #pragma unroll

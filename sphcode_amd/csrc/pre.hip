@@ -86,14 +86,13 @@ __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, int n, PreA
     using Kern = typename KernelOf<D, K, T>::type;
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if(i >= n) return;
-    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
     const int cnt = a.cnt[i];
     const T h = (T)s.f[F_SML][i];
     const Kern ker(h);
     Acc<T> rho;
     for(int k = 0; k < cnt; ++k) {
-        const T r = list_r<T>(a, base + (size_t)k * WAVE);
-        if(r < h) rho.add(fld<T>(s, F_MASS, a.list[base + (size_t)k * WAVE]) * ker.w(ker.q_of(r)));
+        const T r = list_r<T>(a, list_at(i, k, a.cap));
+        if(r < h) rho.add(fld<T>(s, F_MASS, a.list[list_at(i, k, a.cap)]) * ker.w(ker.q_of(r)));
     }
     s.f[F_DENS][i] = rho.val();
 }
@@ -108,7 +107,6 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
     double hvs = 1.7976931348623157e308;
     int64_t iters = 0, fails = 0, trunc = 0, pairs = 0;
     if(valid) {
-        const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
         const int cnt = a.cnt[i];
         const double mi = s.f[F_MASS][i];
         const double Hs = s.f[F_SML][i];  // search radius h_guess * ratio
@@ -125,14 +123,14 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 const Kern ker(ht);
                 Acc<T> rho, drho;
                 for(int k = 0; k < cnt; ++k) {
-                    const T r = list_r<T>(a, base + (size_t)k * WAVE);
+                    const T r = list_r<T>(a, list_at(i, k, a.cap));
                     if(r < ht) {
                         const T q = ker.q_of(r);
                         if constexpr(NUMBER) {
                             rho.add(ker.w(q));
                             drho.add(ker.dhw(q));
                         } else {
-                            const T mj = fld<T>(s, F_MASS, a.list[base + (size_t)k * WAVE]);
+                            const T mj = fld<T>(s, F_MASS, a.list[list_at(i, k, a.cap)]);
                             rho.add(mj * ker.w(q));
                             drho.add(mj * ker.dhw(q));
                         }
@@ -169,9 +167,9 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
         for(int k = 0; k < cnt; ++k) {
-            const T r = list_r<T>(a, base + (size_t)k * WAVE);
+            const T r = list_r<T>(a, list_at(i, k, a.cap));
             if(!(r < ht)) continue;
-            const int j = a.list[base + (size_t)k * WAVE];
+            const int j = a.list[list_at(i, k, a.cap)];
             ++count;
             const T mj = fld<T>(s, F_MASS, j);
             const T q = ker.q_of(r);

@@ -26,8 +26,7 @@
 // the list and the rest at the back, so that one walk serves both the
 // pre-interaction and the force stage.
 //
-// List layout: entry k of the particle in slot (wave w, lane l) lives at
-// (w * cap + k) * 64 + l, so a wave's k-th loads are one contiguous line.
+// List layout: list_at (ctx.hpp).
 #include "ctx.hpp"
 
 namespace sphgpu {
@@ -136,10 +135,9 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     }
     const double hmax = wave_maxv(valid ? h : 0.0);
 
-    const size_t base = ((size_t)i / WAVE) * a.cap * WAVE + i % WAVE;
-    int *out = a.list + base;
-    double *outr = a.list_r ? a.list_r + base : nullptr;
-    float *outrf = a.list_rf ? a.list_rf + base : nullptr;
+    int *out = a.list;
+    double *outr = a.list_r;
+    float *outrf = a.list_rf;
     int count = 0, count2 = 0;  // candidates found (front / back)
     int wf = 0, wb = 0;          // of those, written (at most cap in total)
 
@@ -284,15 +282,16 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                         const bool room = wf + wb < a.cap;
                         if(!a.split || r2 < h2_own) {
                             if(room) {
-                                out[(size_t)wf * WAVE] = c_j[w][c];
-                                if(outr) outr[(size_t)wf * WAVE] = sqrt(r2);
-                                if(outrf) outrf[(size_t)wf * WAVE] = (float)sqrt(r2);
+                                const size_t at = list_at(i, wf, a.cap);
+                                out[at] = c_j[w][c];
+                                if(outr) outr[at] = sqrt(r2);
+                                if(outrf) outrf[at] = (float)sqrt(r2);
                                 ++wf;
                             }
                             ++count;
                         } else {
                             if(room) {
-                                out[(size_t)(a.cap - 1 - wb) * WAVE] = c_j[w][c];
+                                out[list_at(i, a.cap - 1 - wb, a.cap)] = c_j[w][c];
                                 ++wb;
                             }
                             ++count2;
