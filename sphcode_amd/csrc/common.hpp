@@ -30,6 +30,12 @@ __device__ __forceinline__ int wave_min_int(int v)
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 // MI355X deals workgroups round-robin over its 8 XCDs, each with its own L2.
 // Renumbering the launch so that XCD x runs one contiguous range of
@@ -162,10 +168,13 @@ template <int D> __device__ __forceinline__ void wrap(const BoxDev &b, double x[
 
 // Cubic spline with support h: H = h/2, q = r/H < 2.
 // 1/sqrt(x) in the pair arithmetic type (HIP's rsqrt takes double only, so
-// a float argument would be widened and refined in FP64)
+// a float argument would be widened and refined in FP64).  The FP32 form is
+// the hardware reciprocal square root (about 1 ulp) without the rescaling
+// of denormal inputs: the separations it is applied to are far above
+// FLT_MIN (the caller excludes zero).
 template <class T> __device__ __forceinline__ T rsqrt_t(T x)
 {
-    if constexpr(sizeof(T) == 4) return rsqrtf(x);
+    if constexpr(sizeof(T) == 4) return __builtin_amdgcn_rsqf(x);
     else return rsqrt(x);
 }
 

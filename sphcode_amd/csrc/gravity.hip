@@ -32,77 +32,70 @@ template <int D, class T> struct GravityVisitor {
     T einv_i;   // 2 / h_i
     Acc<T> phi;
     Acc<T> acc[D];
-    double d[D], d2 = 0.0;
-    int npp = 0, npn = 0, nopen = 0;  // per lane, well below 2^31
 
     __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_) {}
 
-    __device__ __forceinline__ bool open(int node)
+    // Opening test edge^2 > theta^2 |x_i - com|^2 (FP64, all lanes); a lane
+    // of `me` that does not open adds the monopole, the others add zero.
+    __device__ __forceinline__ bool node(int n, bool me)
     {
-        const double4 g = t.geo[node];
-        const double4 m = t.mom[node];
-        const double l2 = g.w * g.w;
+        const double4 g = t.geo[n];
+        const double4 m = t.mom[n];
+        double d[D];
         d[0] = xi[0] - m.x;
         if constexpr(D > 1) d[1] = xi[1] - m.y;
         if constexpr(D > 2) d[2] = xi[2] - m.z;
         min_image<D>(box, d);
-        d2 = d[0] * d[0];
+        double d2 = d[0] * d[0];
 #pragma unroll
         for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
-        const bool o = l2 > theta2 * d2;
-        nopen += o ? 1 : 0;
-        return o;
-    }
-
-    __device__ __forceinline__ void far(int node)
-    {
-        const T gm = (T)(G * t.mom[node].w);
-        const T rinv = rsqrt_t<T>((T)d2);
+        const bool o = g.w * g.w > theta2 * d2;
+        const bool far = me && !o;
+        const T gm = far ? (T)(G * m.w) : T(0);
+        const T rinv = rsqrt_t<T>(far ? (T)d2 : T(1));
         phi.add(-(gm * rinv));
         const T c = gm * rinv * rinv * rinv;
 #pragma unroll
         for(int k = 0; k < D; ++k) acc[k].add(-((T)d[k] * c));
-        ++npn;
+        return me && o;
     }
 
     __device__ __forceinline__ void leaf(int4 r, bool open)
     {
+        const T w = open ? T(1) : T(0);
         for(int j = r.x; j < r.y; ++j) {
             double xj[D];
 #pragma unroll
             for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
             const T mj = fld<T>(s, F_MASS, j);
             const T einv_j = T(2) * fld<T>(s, F_HINV, j);
-            if(open) {
-                double dd[D];
-                const double r2d = sep2<D>(box, xi, xj, dd);
-                T rij[D];
-                T r2 = T(0);
+            double dd[D];
+            const double r2d = sep2<D>(box, xi, xj, dd);
+            T rij[D];
+            T r2 = T(0);
 #pragma unroll
-                for(int k = 0; k < D; ++k) {
-                    rij[k] = (T)dd[k];
-                    r2 += rij[k] * rij[k];
-                }
-                if constexpr(sizeof(T) == 8) r2 = r2d;
-                const T rinv = r2 > T(0) ? rsqrt_t<T>(r2) : T(0);
-                const T rr = r2 * rinv;
-                // far field of both softening lengths unless one is within 2e
-                T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
-                const T ui = rr * einv_i, uj = rr * einv_j;
-                if(ui < T(2) || uj < T(2)) {
-                    T pi_, ai, pj, aj;
-                    soft_both(ui, einv_i, rinv, pi_, ai);
-                    soft_both(uj, einv_j, rinv, pj, aj);
-                    pot = pi_ + pj;
-                    acc3 = ai + aj;
-                }
-                const T gm = (T)G * mj * T(0.5);
-                phi.add(-(gm * pot));
-                const T c = gm * acc3;
-#pragma unroll
-                for(int k = 0; k < D; ++k) acc[k].add(-(rij[k] * c));
-                ++npp;
+            for(int k = 0; k < D; ++k) {
+                rij[k] = (T)dd[k];
+                r2 += rij[k] * rij[k];
             }
+            if constexpr(sizeof(T) == 8) r2 = r2d;
+            const T rinv = r2 > T(0) ? rsqrt_t<T>(r2) : T(0);
+            const T rr = r2 * rinv;
+            // far field of both softening lengths unless one is within 2e
+            T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
+            const T ui = rr * einv_i, uj = rr * einv_j;
+            if(ui < T(2) || uj < T(2)) {
+                T pi_, ai, pj, aj;
+                soft_both(ui, einv_i, rinv, pi_, ai);
+                soft_both(uj, einv_j, rinv, pj, aj);
+                pot = pi_ + pj;
+                acc3 = ai + aj;
+            }
+            const T gm = (T)G * mj * T(0.5) * w;
+            phi.add(-(gm * pot));
+            const T c = gm * acc3;
+#pragma unroll
+            for(int k = 0; k < D; ++k) acc[k].add(-(rij[k] * c));
         }
     }
 };
@@ -126,18 +119,17 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kerne
     __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
-    const int visits = tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+    const WalkCount wc = tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
         for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k].val();
     }
-    const int64_t pp = wave_sum((int64_t)v.npp), pn = wave_sum((int64_t)v.npn), po = wave_sum((int64_t)v.nopen);
     if(wave_lane() == 0) {
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)pp);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)pn);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)po);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)visits);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)wc.pairs);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)wc.accepted);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)wc.opened);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)wc.visits);
     }
 }
 

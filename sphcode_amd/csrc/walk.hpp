@@ -23,22 +23,37 @@ namespace sphgpu {
 constexpr int WALK_DONE = 0x7fffffff;
 constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 
-// Visitor interface:
-//   bool open(int node)                  lane test (active lanes only)
-//   void far(int node)                   node accepted without opening
-//   void leaf(int4 rng, bool open)       whole wave, uniform particle loop
-// Returns the number of node visits of the wave (wave-uniform).
+// Visitor interface (every call is made by the whole wave):
+//   bool node(int node, bool me)         the lane's own decision at the node,
+//                                        for lanes with `me`; the visitor
+//                                        sums a node accepted without opening
+//                                        itself, masked rather than branched
+//   void leaf(int4 rng, bool open)       particles of an opened leaf, for the
+//                                        lanes with `open`
+// The walk state (cursor, masks, parked stack) is wave-uniform and kept in
+// scalar registers.  WalkCount returns, per wave, the node visits and the
+// lane totals of opened nodes, accepted nodes and particle interactions.
+struct WalkCount {
+    int64_t visits = 0, opened = 0, accepted = 0, pairs = 0;
+};
+
 template <class V>
-__device__ __forceinline__ int tree_walk(const TreeDev &t, int total, bool valid, V &v, int *stk_node, uint64_t *stk_mask)
+__device__ __forceinline__ WalkCount tree_walk(const TreeDev &t, int total, bool valid, V &v, int *stk_node,
+                                               uint64_t *stk_mask)
 {
-    int visits = 0;
+    WalkCount wc;
     const int lane = wave_lane();
-    uint64_t active = __ballot(valid);
+    uint64_t active = uniform64(__ballot(valid));
     int node = 0;
     int sp = 0;                    // entries below the top, in LDS
-    int top_node = WALK_DONE;      // top entry, kept in scalar registers
+    int top_node = WALK_DONE;      // top entry
     uint64_t top_mask = 0;
     while(true) {
+        active = uniform64(active);
+        node = uniform(node);
+        top_node = uniform(top_node);
+        top_mask = uniform64(top_mask);
+        sp = uniform(sp);
         if(active == 0 || node >= total) {
             if(top_node == WALK_DONE) break;
             node = top_node;
@@ -65,20 +80,21 @@ __device__ __forceinline__ int tree_walk(const TreeDev &t, int total, bool valid
             }
             continue;
         }
-        node = uniform(node);
-        ++visits;
+        ++wc.visits;
         const int4 r = t.rng[node];
         const bool leaf = (r.w & 256) != 0;
         const int skip = r.z > node ? r.z : node + 1;
         const bool me = (active >> lane) & 1ull;
-        bool open = false;
-        if(me) {
-            open = v.open(node);
-            if(!open) v.far(node);
-        }
-        const uint64_t om = __ballot(open);
+        const bool open = v.node(node, me);
+        const uint64_t om = uniform64(__ballot(open));
+        const int nopen = __popcll(om);
+        wc.opened += nopen;
+        wc.accepted += __popcll(active) - nopen;
         if(leaf) {
-            if(om) v.leaf(r, open);
+            if(om) {
+                v.leaf(r, open);
+                wc.pairs += (int64_t)nopen * (r.y - r.x);
+            }
             node = skip;
         } else if(om) {
             const uint64_t fm = active & ~om;
@@ -102,7 +118,7 @@ __device__ __forceinline__ int tree_walk(const TreeDev &t, int total, bool valid
             node = skip;
         }
     }
-    return visits;
+    return wc;
 }
 
 }  // namespace sphgpu

@@ -134,11 +134,18 @@ def test_three_steps(gpu, case):
     print(f"[steps] case={case} n={len(f['mass'])}", flush=True)
     ref.initialize()
     gpu_e.initialize()
+    e0a, e0b = ref.energy()[3], gpu_e.energy()[3]
     for _ in range(3):
         ref.integrate()
         gpu_e.integrate()
         ta, tb = ref.time(), gpu_e.time()
         assert abs(tb[1] - ta[1]) <= 1e-9 * ta[1], f"{case}: dt {tb[1]} vs {ta[1]}"
+    # total-energy drift over the three steps (src/output.cpp:66-90 sums)
+    ea, eb = ref.energy(), gpu_e.energy()
+    drift_a, drift_b = ea[3] - e0a, eb[3] - e0b
+    scale = abs(ea[0]) + abs(ea[1]) + abs(ea[2])
+    print(f"[steps] {case}: energy drift oracle {drift_a:.6e} gpu {drift_b:.6e} (scale {scale:.3e})", flush=True)
+    assert abs(drift_b - drift_a) <= 1e-6 * abs(drift_a) + 1e-10 * scale, f"{case}: drift {drift_b} vs {drift_a}"
     a = ref.download()
     b = gpu_e.download()
     for k in ("sml", "dens", "pres"):
