@@ -210,6 +210,15 @@ int sphgpu_integrate(void *ctx, double fixed_dt);
 int sphgpu_get_time(void *ctx, double *time, double *dt, double *h_per_v_sig);
 int sphgpu_set_time(void *ctx, double time, double dt);
 
+/* Resume a run from a state saved after a full step (the reference has no
+   restart; its output is lossy text, src/output.cpp:14-64).  After
+   sphgpu_upload of the saved particles (h, density, alpha, acceleration and
+   du/dt included), this sets the time, dt and min h/v_sig the next
+   sphgpu_integrate uses, skips the one-off initial smoothing and builds the
+   tree.  A continued run then matches the uninterrupted one to round-off
+   (particles in the same octree cell may be summed in another order). */
+int sphgpu_restart(void *ctx, double time, double dt, double h_per_v_sig);
+
 /* Global FP64 reductions of the reference's energy output
    (src/output.cpp:66-90): out[0]=kinetic, out[1]=thermal,
    out[2]=potential (sum of m phi / 2), out[3]=total, out[4..4+dim) =

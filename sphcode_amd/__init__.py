@@ -52,6 +52,10 @@ class GPUEngine(Engine):
     def synchronize(self) -> None:
         self._call("synchronize")
 
+    def restart(self, time: float, dt: float, h_per_v_sig: float) -> None:
+        """Resume from an uploaded state saved after a full step."""
+        self._call("restart", C.c_double(time), C.c_double(dt), C.c_double(h_per_v_sig))
+
     # ---------------------------------------------------- differentiable rollout
     def rollout(self, steps: int, dt: float) -> None:
         """Solver::initialize, then ``steps`` steps of fixed ``dt``, with

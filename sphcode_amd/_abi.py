@@ -195,6 +195,7 @@ GPU_ONLY = {
     "enable_timers": (C.c_int, [C.c_void_p, C.c_int]),
     "synchronize": (C.c_int, [C.c_void_p]),
     "rollout": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "restart": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "rollout_adjoint": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP, _DP, _DP]),
 }
 

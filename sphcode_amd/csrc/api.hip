@@ -674,6 +674,25 @@ int sphgpu_set_time(void *ptr, double time, double dt)
     return SPHGPU_OK;
 }
 
+// Resume from a state saved after a full step: the uploaded particles keep
+// their h, density (the next guess), alpha, acceleration and du/dt; the
+// one-off initial smoothing is skipped and the tree is built.
+int sphgpu_restart(void *ptr, double time, double dt, double h_per_v_sig)
+{
+    SPHGPU_CTX(ptr);
+    SPHGPU_NEED_DATA();
+    if(!(h_per_v_sig > 0.0)) return set_error(SPHGPU_E_INVALID, "restart: h_per_v_sig must be > 0");
+    c.time = time;
+    c.dt = dt;
+    c.h_per_v_sig = h_per_v_sig;
+    hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, dt);
+    hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_HVS, h_per_v_sig);
+    SPHGPU_HIP(hipGetLastError());
+    c.first_pre = false;
+    int rc = stage_tree(c);
+    return rc ? rc : check_flags(c, "restart");
+}
+
 int sphgpu_energy(void *ptr, double *out)
 {
     SPHGPU_CTX(ptr);

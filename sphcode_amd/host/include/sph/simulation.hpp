@@ -41,6 +41,9 @@ public:
     double get_h_per_v_sig() const;
     void set_dt(double dt);
     void update_time();  // time += dt (src/simulation.cpp:32-35)
+    // continue from a state saved after a full step: time, dt and min h/v_sig
+    // as saved, no initial smoothing, tree rebuilt (sphgpu_restart)
+    void restart(double time, double dt, double h_per_v_sig);
 
     void make_tree();    // BHTree::make on the device (src/simulation.cpp:37-40)
 

@@ -26,6 +26,11 @@ public:
     explicit Solver(std::shared_ptr<SPHParameters> param);
     // Solver::initialize after the initial condition (src/solver.cpp:353-415)
     void initialize(const std::vector<SPHParticle> &particles);
+    // In place of initialize: continue from particles saved after a full
+    // step (every field of SPHParticle as written) at the saved time, dt and
+    // min h/v_sig.  The reference has no restart (its output is lossy text,
+    // src/output.cpp:14-64).
+    void restart(const std::vector<SPHParticle> &particles, double time, double dt, double h_per_v_sig);
     // Solver::integrate + update_time (src/solver.cpp:417-429)
     void integrate();
     // integrate() until endTime; returns the number of steps

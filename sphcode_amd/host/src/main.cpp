@@ -5,9 +5,12 @@
 // in the reference.
 //
 //   sphgpu_run PARAMS.json --print-params
-//   sphgpu_run PARAMS.json --particles IN.snap [--steps K] [--output OUT.snap] [--energy]
+//   sphgpu_run PARAMS.json --particles IN.snap [--restart] [--steps K] [--output OUT.snap] [--energy]
 //
 // Without --steps the run goes to endTime (the loop of src/solver.cpp:318).
+// With --restart IN.snap is a state written by --output after a full step:
+// the run continues from its time, dt and min h/v_sig instead of
+// initialising (the reference has no restart).
 
 #include <cstdio>
 #include <cstdlib>
@@ -58,8 +61,8 @@ void print_energy(const sph::Simulation &sim, int loop) {
 
 int usage() {
     std::fprintf(stderr, "usage: sphgpu_run PARAMS.json --print-params\n"
-                         "       sphgpu_run PARAMS.json --particles IN.snap [--steps K] [--output OUT.snap] "
-                         "[--energy]\n");
+                         "       sphgpu_run PARAMS.json --particles IN.snap [--restart] [--steps K] "
+                         "[--output OUT.snap] [--energy]\n");
     return 2;
 }
 
@@ -70,10 +73,11 @@ int main(int argc, char **argv) {
     const std::string param_file = argv[1];
     std::string particles, output;
     int steps = -1;
-    bool show = false, energy = false;
+    bool show = false, energy = false, restart = false;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--print-params")) show = true;
         else if (!std::strcmp(argv[a], "--energy")) energy = true;
+        else if (!std::strcmp(argv[a], "--restart")) restart = true;
         else if (!std::strcmp(argv[a], "--particles") && a + 1 < argc) particles = argv[++a];
         else if (!std::strcmp(argv[a], "--output") && a + 1 < argc) output = argv[++a];
         else if (!std::strcmp(argv[a], "--steps") && a + 1 < argc) steps = std::atoi(argv[++a]);
@@ -90,7 +94,12 @@ int main(int argc, char **argv) {
         sph::Snapshot in = sph::read_snapshot(particles);
         if (in.dim != param->dim) throw sph::SPHException("snapshot DIM differs from the parameter file's DIM");
         sph::Solver solver(param);
-        solver.initialize(in.particles);
+        if (restart) {
+            if (!(in.h_per_v_sig > 0.0)) throw sph::SPHException("--restart needs a snapshot written after a step");
+            solver.restart(in.particles, in.time, in.dt, in.h_per_v_sig);
+        } else {
+            solver.initialize(in.particles);
+        }
         auto sim = solver.simulation();
         if (energy) print_energy(*sim, 0);
         int loop = 0;
@@ -109,6 +118,7 @@ int main(int argc, char **argv) {
             out.dim = param->dim;
             out.time = sim->get_time();
             out.dt = sim->get_dt();
+            out.h_per_v_sig = sim->get_h_per_v_sig();
             out.particles = sim->get_particles();
             sph::write_snapshot(output, out);
         }

@@ -145,6 +145,10 @@ void Simulation::update_time() {
     check_status(sphgpu_set_time(m_ctx, t + dt, dt), "sphgpu_set_time");
 }
 
+void Simulation::restart(double time, double dt, double h_per_v_sig) {
+    check_status(sphgpu_restart(m_ctx, time, dt, h_per_v_sig), "sphgpu_restart");
+}
+
 void Simulation::make_tree() { check_status(sphgpu_make_tree(m_ctx), "sphgpu_make_tree"); }
 
 std::array<double, 7> Simulation::energy() const {

@@ -10,7 +10,8 @@ namespace sph {
 
 namespace {
 
-constexpr char MAGIC[8] = {'S', 'P', 'H', 'S', 'N', 'A', 'P', '1'};
+constexpr char MAGIC[8] = {'S', 'P', 'H', 'S', 'N', 'A', 'P', '2'};
+constexpr char MAGIC1[8] = {'S', 'P', 'H', 'S', 'N', 'A', 'P', '1'};
 
 template <class T> void put(std::ofstream &f, const T &v) { f.write(reinterpret_cast<const char *>(&v), sizeof(T)); }
 
@@ -42,6 +43,7 @@ void write_snapshot(const std::string &path, const Snapshot &s) {
     put<int64_t>(f, (int64_t)s.particles.size());
     put<double>(f, s.time);
     put<double>(f, s.dt);
+    put<double>(f, s.h_per_v_sig);
     for (VecField m : kVec)
         for (const auto &p : s.particles)
             for (int k = 0; k < s.dim; ++k) put<double>(f, (p.*m)[k]);
@@ -56,7 +58,9 @@ Snapshot read_snapshot(const std::string &path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) throw SPHException(path + ": cannot open file");
     char magic[8];
-    if (!f.read(magic, 8) || std::memcmp(magic, MAGIC, 8) != 0) throw SPHException(path + ": not a snapshot");
+    if (!f.read(magic, 8)) throw SPHException(path + ": not a snapshot");
+    const bool v2 = std::memcmp(magic, MAGIC, 8) == 0;
+    if (!v2 && std::memcmp(magic, MAGIC1, 8) != 0) throw SPHException(path + ": not a snapshot");
     Snapshot s;
     s.dim = get<int32_t>(f, path);
     (void)get<int32_t>(f, path);
@@ -64,6 +68,7 @@ Snapshot read_snapshot(const std::string &path) {
     if (s.dim < 1 || s.dim > 3 || n < 0 || n > (int64_t)1 << 31) throw SPHException(path + ": bad snapshot header");
     s.time = get<double>(f, path);
     s.dt = get<double>(f, path);
+    if (v2) s.h_per_v_sig = get<double>(f, path);
     s.particles.resize((size_t)n);
     for (VecField m : kVec)
         for (auto &p : s.particles)

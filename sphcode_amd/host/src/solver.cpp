@@ -48,6 +48,12 @@ void Solver::initialize(const std::vector<SPHParticle> &particles) {
     m_gforce->calculation(m_sim);
 }
 
+void Solver::restart(const std::vector<SPHParticle> &particles, double time, double dt, double h_per_v_sig) {
+    m_sim = std::make_shared<Simulation>(m_param);
+    m_sim->set_particles(particles);
+    m_sim->restart(time, dt, h_per_v_sig);
+}
+
 // src/solver.cpp:417-429 and the time update of Solver::run (:327)
 void Solver::integrate() {
     if (!m_sim) throw SPHException("Solver::integrate before initialize");

@@ -149,28 +149,29 @@ def evrard(n: int = 1 << 20, seed: int = 1, jitter: float = 0.05, leaf: int = 32
     return p, fields
 
 
-def rollout(n_side: int = 64, seed: int = 1, precision: int = _abi.FP64):
-    """3D periodic unit box for the differentiable rollout.
+def rollout(n_side: int = 64, seed: int = 1, precision: int = _abi.FP64, dim: int = 3):
+    """Periodic unit box for the differentiable rollout (3D by default).
 
-    n_side^3 lattice with Gaussian jitter of 0.05 spacings, rho = 1,
+    n_side^dim lattice with Gaussian jitter of 0.05 spacings, rho = 1,
     u = 1 (1 + 0.1 (2U - 1)), velocities N(0, 0.1^2) per axis, gamma = 5/3,
-    SSPH with the cubic spline and 50 neighbours.
+    SSPH with the cubic spline and 50 neighbours in 3D (24 in 2D, 6 in 1D).
     """
     gamma = 5.0 / 3.0
     rng = np.random.default_rng(seed)
     dx = 1.0 / n_side
     g = (np.arange(n_side) + 0.5) * dx
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    grids = np.meshgrid(*([g] * dim), indexing="ij")
+    pos = np.stack([a.ravel() for a in grids], axis=1)
     pos = np.mod(pos + rng.normal(0.0, 0.05 * dx, pos.shape), 1.0)
     n = pos.shape[0]
     mass = np.full(n, 1.0 / n)
     dens = np.ones(n)
     ene = 1.0 + 0.1 * (2.0 * rng.random(n) - 1.0)
     pres = (gamma - 1.0) * dens * ene
-    vel = rng.normal(0.0, 0.1, (n, 3))
-    p = _base(3, sph_type=_abi.SSPH, kernel=_abi.CUBIC_SPLINE, neighbor_number=50, gamma=gamma,
-              periodic_is_valid=1, range_max=(1.0, 1.0, 1.0), range_min=(0.0, 0.0, 0.0),
+    vel = rng.normal(0.0, 0.1, (n, dim))
+    nngb = {1: 6, 2: 24, 3: 50}[dim]
+    p = _base(dim, sph_type=_abi.SSPH, kernel=_abi.CUBIC_SPLINE, neighbor_number=nngb, gamma=gamma,
+              periodic_is_valid=1, range_max=(1.0,) * dim, range_min=(0.0,) * dim,
               time_end=1.0, precision=precision)
     fields = dict(pos=pos, vel=vel, mass=mass, dens=dens, pres=pres, ene=ene)
     return p, fields

@@ -1,8 +1,9 @@
 """Full-precision binary snapshots and JSON parameter files for sphgpu_run.
 
 The snapshot layout is the one the C++ host layer reads and writes
-(little endian): ``b"SPHSNAP1"``, int32 dim, int32 0, int64 n, float64 time,
-float64 dt, then pos, vel, vel_p, acc (n*dim float64 each), mass, dens, pres,
+(little endian): ``b"SPHSNAP2"``, int32 dim, int32 0, int64 n, float64 time,
+float64 dt, float64 h_per_v_sig (absent in ``b"SPHSNAP1"`` files), then pos,
+vel, vel_p, acc (n*dim float64 each), mass, dens, pres,
 ene, ene_p, dene, sml, sound, balsara, alpha, gradh, phi (n float64 each), id,
 neighbor (n int32 each).  It replaces the reference's lossy text snapshots
 (src/output.cpp:14-64).
@@ -19,21 +20,22 @@ import numpy as np
 
 from . import _abi
 
-MAGIC = b"SPHSNAP1"
+MAGIC = b"SPHSNAP2"
+MAGIC1 = b"SPHSNAP1"
 VEC_FIELDS = ("pos", "vel", "vel_p", "acc")
 SCALAR_FIELDS = ("mass", "dens", "pres", "ene", "ene_p", "dene", "sml", "sound",
                  "balsara", "alpha", "gradh", "phi")
 INT_FIELDS = ("id", "neighbor")
 
 
-def write(path, fields: dict, dim: int, time: float = 0.0, dt: float = 0.0) -> None:
+def write(path, fields: dict, dim: int, time: float = 0.0, dt: float = 0.0, h_per_v_sig: float = 0.0) -> None:
     """Write a snapshot; absent fields are written as zeros (id as 0..n-1)."""
     n = int(np.asarray(fields["pos"]).shape[0])
     with open(path, "wb") as f:
         f.write(MAGIC)
         f.write(np.array([dim, 0], "<i4").tobytes())
         f.write(np.array([n], "<i8").tobytes())
-        f.write(np.array([time, dt], "<f8").tobytes())
+        f.write(np.array([time, dt, h_per_v_sig], "<f8").tobytes())
         for k in VEC_FIELDS:
             a = fields.get(k)
             a = np.zeros((n, dim)) if a is None else np.asarray(a, dtype=np.float64).reshape(n, dim)
@@ -49,17 +51,20 @@ def write(path, fields: dict, dim: int, time: float = 0.0, dt: float = 0.0) -> N
             f.write(np.ascontiguousarray(np.asarray(a).reshape(n), dtype="<i4").tobytes())
 
 
-def read(path) -> tuple[dict, int, float, float]:
-    """Read a snapshot: (fields, dim, time, dt)."""
+def read(path, with_hvs: bool = False):
+    """Read a snapshot: (fields, dim, time, dt), plus h_per_v_sig last if
+    ``with_hvs``."""
     with open(path, "rb") as f:
         buf = f.read()
-    if buf[:8] != MAGIC:
+    if buf[:8] not in (MAGIC, MAGIC1):
         raise ValueError(f"{path}: not a snapshot")
     dim, _ = np.frombuffer(buf, "<i4", 2, 8)
     (n,) = np.frombuffer(buf, "<i8", 1, 16)
     time, dt = np.frombuffer(buf, "<f8", 2, 24)
+    v2 = buf[:8] == MAGIC
+    hvs = float(np.frombuffer(buf, "<f8", 1, 40)[0]) if v2 else 0.0
     dim, n = int(dim), int(n)
-    off = 40
+    off = 48 if v2 else 40
     out = {}
     for k in VEC_FIELDS:
         out[k] = np.frombuffer(buf, "<f8", n * dim, off).reshape(n, dim).copy()
@@ -72,6 +77,8 @@ def read(path) -> tuple[dict, int, float, float]:
         off += 4 * n
     if off != len(buf):
         raise ValueError(f"{path}: {len(buf) - off} trailing bytes")
+    if with_hvs:
+        return out, dim, float(time), float(dt), hvs
     return out, dim, float(time), float(dt)
 
 

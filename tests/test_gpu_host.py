@@ -96,3 +96,52 @@ def test_cli_reports_stage_errors(gpu, tmp_path):
     r = subprocess.run([str(RUN), str(pj), "--particles", str(sin), "--steps", "1"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "error:" in r.stderr and "capacity" in r.stderr.lower(), r.stderr
+
+
+def _cli(tmp_path, pj, sin, sout, steps, restart):
+    cmd = [str(RUN), str(pj), "--particles", str(sin), "--output", str(sout), "--steps", str(steps)]
+    if restart:
+        cmd.append("--restart")
+    print("sphgpu_run", " ".join(cmd[1:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return snapshot.read(sout, with_hvs=True)
+
+
+@pytest.mark.parametrize("case", ["sod_ssph", "sod_gsph2", "khi_disph", "evrard_disph"])
+def test_restart_continues_run(gpu, tmp_path, case):
+    """A run of 3 steps written with --output and continued with --restart
+    for 3 more reaches the state of 6 uninterrupted steps: the snapshot keeps
+    every FP64 bit plus time, dt and min h/v_sig, and the restart skips the
+    initial smoothing.  The continued run sorts the particles from caller
+    order rather than from the last step's order, so particles that share a
+    Morton key may be summed in another order: the bar is round-off."""
+    p, f = CASES[case]()
+    p.time_end = 1.0
+    pj, sin = tmp_path / "p.json", tmp_path / "in.snap"
+    snapshot.write_params(pj, p)
+    snapshot.write(sin, f, int(p.dim))
+    want, _, t6, dt6, hv6 = _cli(tmp_path, pj, sin, tmp_path / "six.snap", 6, False)
+    mid, _, t3, _, hv3 = _cli(tmp_path, pj, sin, tmp_path / "three.snap", 3, False)
+    assert hv3 > 0 and 0 < t3 < t6
+    got, _, t, dt, hv = _cli(tmp_path, pj, tmp_path / "three.snap", tmp_path / "cont.snap", 3, True)
+    np.testing.assert_allclose(t, t6, rtol=1e-12)
+    np.testing.assert_allclose(dt, dt6, rtol=1e-10)
+    np.testing.assert_allclose(hv, hv6, rtol=1e-10)
+    np.testing.assert_array_equal(got["id"], want["id"])
+    for k in ("pos", "vel", "ene", "sml", "dens", "pres", "alpha", "acc", "dene"):
+        scale = np.abs(want[k]).max() + 1e-300
+        err = np.abs(got[k] - want[k]).max() / scale
+        assert err < 1e-10, f"{case}: {k} max rel err {err:.3e}"
+
+
+def test_restart_rejects_fresh_snapshot(gpu, tmp_path):
+    """--restart needs the min h/v_sig of a finished step; an initial
+    condition written without it is refused."""
+    p, f = CASES["sod_ssph"]()
+    pj, sin = tmp_path / "p.json", tmp_path / "in.snap"
+    snapshot.write_params(pj, p)
+    snapshot.write(sin, f, int(p.dim))
+    r = subprocess.run([str(RUN), str(pj), "--particles", str(sin), "--steps", "1", "--restart"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "--restart" in r.stderr, r.stderr

@@ -147,4 +147,4 @@ def test_snapshot_round_trip(tmp_path):
     assert (d, t, dt) == (dim, 0.125, 1e-3)
     for k, v in fields.items():
         np.testing.assert_array_equal(got[k], v)
-    assert f.stat().st_size == 40 + n * (8 * (4 * dim + 12) + 8)
+    assert f.stat().st_size == 48 + n * (8 * (4 * dim + 12) + 8)
