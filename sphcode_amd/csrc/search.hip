@@ -1,8 +1,9 @@
 // Neighbour search (BHTree::neighbor_search, src/bhtree.cpp:114-126,
 // 234-270) for 64 Morton-consecutive particles per wave, in two phases:
 //
-//  1. Group walk.  The wave walks the pre-order tree once with a single,
-//     wave-uniform cursor, keeping every leaf whose cube can hold a
+//  1. Group walk.  The wave walks the pre-order tree once, 64 consecutive
+//     nodes at a time (one per lane, visited ones found by a prefix maximum
+//     of subtree ends), keeping every leaf whose cube can hold a
 //     candidate of any of its particles: per axis, the minimum-image
 //     distance between the group's box centre and the cell centre must not
 //     exceed group half-extent + edge/2 + H, with H the group's largest
@@ -14,7 +15,8 @@
 //     every lane then applies the reference's leaf test to each staged
 //     particle: |x_i - x_j|^2 < H_i^2 with H_i = h_guess * kernel_ratio
 //     (gather, src/pre_interaction.cpp:61-70) or max(h_i, leaf's largest h)
-//     (symmetric, src/bhtree.cpp:237, 256).
+//     (symmetric, src/bhtree.cpp:237, 256), into a 64-bit hit mask, and
+//     appends its hits in staging order.
 //
 // The sets are the reference's; the reference then sorts each list by
 // distance (src/bhtree.cpp:120-124) and stops at the first r >= h, whereas
@@ -145,19 +147,26 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     int node = 0;
     int64_t visited = 0, tested = 0;
     while(node < total) {
-        // ---- phase 1: collect up to LEAFCAP leaves.  The lanes test a
-        // window of 64 consecutive pre-order nodes at once; the visited
-        // nodes of the window are then the chain node -> next(node) ->
-        // ..., next = node+1 for an overlapping internal node and skip
-        // otherwise, followed with scalar reads of the lanes' results.
+        // ---- phase 1: collect leaves, 64 consecutive pre-order nodes per
+        // window.  A node is visited iff every ancestor of it is opened
+        // (overlapping and internal).  Within a window the window's first
+        // node is visited, and a later node v is visited iff no earlier
+        // node of the window that is not opened has its subtree span
+        // [u, skip(u)) over v: v >= the exclusive prefix maximum of skip
+        // over the window's not-opened nodes.  The next window starts at
+        // the larger of the window's end and that maximum at the last lane.
+        // Leaves are kept in pre-order, as the one-node-at-a-time walk
+        // would keep them.
         int nleaf = 0;
-        while(node < total && nleaf < LEAFCAP) {
+        while(node < total && nleaf <= LEAFCAP - WAVE) {
             node = uniform(node);
             const int v = node + lane;
-            int flags = 0, nxt = v + 1;
+            const bool in = v < total;
+            int closed_skip = 0;
+            bool leafhit = false;
             int4 r = make_int4(0, 0, 0, 0);
             double ks = 0.0;
-            if(v < total) {
+            if(in) {
                 const double4 g = t.geo[v];
                 r = t.rng[v];
                 double H = hmax;
@@ -175,34 +184,31 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                 }
                 const bool leaf = (r.w & 256) != 0;
                 const int skip = r.z > v ? r.z : v + 1;
-                nxt = overlap && !leaf ? v + 1 : skip;
-                flags = overlap && leaf ? 1 : 0;
+                closed_skip = overlap && !leaf ? 0 : skip;
+                leafhit = overlap && leaf;
             }
-            const int wend = node + WAVE < total ? node + WAVE : total;
-            int c = node;
-            while(c < wend && nleaf < LEAFCAP) {
-                const int l = c - node;
-                ++visited;
-                if(__builtin_amdgcn_readlane(flags, l)) {
-                    const int b = __builtin_amdgcn_readlane(r.x, l);
-                    const int e = __builtin_amdgcn_readlane(r.y, l);
-                    double ksl = 0.0;
-                    if constexpr(IJ) {
-                        const long long bits = __double_as_longlong(ks);
-                        const int lo32 = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), l);
-                        const int hi32 = __builtin_amdgcn_readlane((int)(bits >> 32), l);
-                        ksl = __longlong_as_double(((long long)hi32 << 32) | (unsigned int)lo32);
-                    }
-                    if(lane == 0) {
-                        l_begin[w][nleaf] = b;
-                        l_pre[w][nleaf] = e - b;
-                        l_ks[w][nleaf] = ksl;
-                    }
-                    ++nleaf;
-                }
-                c = __builtin_amdgcn_readlane(nxt, l);
+            // inclusive prefix maximum of closed_skip over the lanes
+            int incl = closed_skip;
+#pragma unroll
+            for(int off = 1; off < WAVE; off <<= 1) {
+                const int y = __shfl_up(incl, off, WAVE);
+                if(lane >= off) incl = incl > y ? incl : y;
             }
-            node = c;
+            int excl = __shfl_up(incl, 1, WAVE);
+            if(lane == 0) excl = 0;
+            const bool vis = in && v >= excl;
+            visited += __popcll(__ballot(vis));
+            const bool hit = vis && leafhit;
+            const uint64_t hits = __ballot(hit);
+            if(hit) {
+                const int at = nleaf + __popcll(hits & ((1ull << lane) - 1ull));
+                l_begin[w][at] = r.x;
+                l_pre[w][at] = r.y - r.x;
+                l_ks[w][at] = ks;
+            }
+            nleaf += uniform(__popcll(hits));
+            const int far = uniform(__shfl(incl, WAVE - 1, WAVE));
+            node = node + WAVE > far ? node + WAVE : far;
         }
         wave_lds_sync();
         // prefix of leaf sizes (each lane owns two slots)
@@ -264,6 +270,9 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
             tested += m;
             wave_lds_sync();
             if(valid) {
+                // test every staged candidate (uniform loop, broadcast LDS
+                // reads), then append this lane's hits in staging order
+                uint64_t mine = 0;
                 for(int c = 0; c < m; ++c) {
                     double xj[D], d[D];
 #pragma unroll
@@ -278,24 +287,32 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                         const double Hc = h < kl ? kl : h;
                         lim = Hc * Hc;
                     }
-                    if(r2 < lim) {
-                        const bool room = wf + wb < a.cap;
-                        if(!a.split || r2 < h2_own) {
-                            if(room) {
-                                const size_t at = list_at(i, wf, a.cap);
-                                out[at] = c_j[w][c];
-                                if(outr) outr[at] = sqrt(r2);
-                                if(outrf) outrf[at] = (float)sqrt(r2);
-                                ++wf;
-                            }
-                            ++count;
-                        } else {
-                            if(room) {
-                                out[list_at(i, a.cap - 1 - wb, a.cap)] = c_j[w][c];
-                                ++wb;
-                            }
-                            ++count2;
+                    mine |= r2 < lim ? 1ull << c : 0ull;
+                }
+                while(mine) {
+                    const int c = __builtin_ctzll(mine);
+                    mine &= mine - 1ull;
+                    double xj[D], d[D];
+#pragma unroll
+                    for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
+                    const double r2 = sep2<D>(box, xi, xj, d);
+                    const int jj = c_j[w][c];
+                    const bool room = wf + wb < a.cap;
+                    if(!a.split || r2 < h2_own) {
+                        if(room) {
+                            const size_t at = list_at(i, wf, a.cap);
+                            out[at] = jj;
+                            if(outr) outr[at] = sqrt(r2);
+                            if(outrf) outrf[at] = (float)sqrt(r2);
+                            ++wf;
                         }
+                        ++count;
+                    } else {
+                        if(room) {
+                            out[list_at(i, a.cap - 1 - wb, a.cap)] = jj;
+                            ++wb;
+                        }
+                        ++count2;
                     }
                 }
             }
