@@ -37,6 +37,12 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// A wave-uniform double moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v)
+{
+    return __longlong_as_double((long long)uniform64((uint64_t)__double_as_longlong(v)));
+}
+
 // MI355X deals workgroups round-robin over its 8 XCDs, each with its own L2.
 // Renumbering the launch so that XCD x runs one contiguous range of
 // logical blocks keeps spatially adjacent particles (adjacent in Morton
@@ -73,6 +79,31 @@ template <class T> __device__ __forceinline__ T wave_maxv(T v)
     }
     return v;
 }
+
+// Inclusive prefix maximum over the 64 lanes of non-negative values, by
+// DPP: row shifts by 1, 2, 4, 8 within each row of 16 lanes, then lane 15's
+// value into row 1 and lane 47's into row 3, then lane 31's into rows 2 and
+// 3.  Lanes whose DPP source is outside the pattern read 0, the identity.
+__device__ __forceinline__ int wave_incl_max_nonneg(int x)
+{
+    int y;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x = y > x ? y : x;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x = y > x ? y : x;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x = y > x ? y : x;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x = y > x ? y : x;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x = y > x ? y : x;
+    y = __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    x = y > x ? y : x;
+    return x;
+}
+
+// Lane l receives lane l-1's value; lane 0 receives 0 (DPP wave_shr:1).
+__device__ __forceinline__ int wave_shr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
 
 // --------------------------------------------------------------- scalar math
 template <class T> __host__ __device__ __forceinline__ T sq(T x) { return x * x; }

@@ -61,6 +61,16 @@ struct TreeDev {
     int *num_nodes;  // device scalar
 };
 
+// Register budgets, as the minimum number of waves per SIMD the compiler
+// must fit (amdgpu_waves_per_eu), for kernels that wait on dependent
+// gathers and where more resident waves beat a few spilled registers
+// (measured; the pre-interaction and force kernels run best at the
+// compiler's own choice).  FP64 gravity keeps the compiler's choice.
+constexpr int OCC_SEARCH = 6;
+template <class T> struct GravityOcc {
+    static constexpr int value = sizeof(T) == 4 ? 8 : 1;
+};
+
 // Neighbour-list addressing.  The lists of 64 consecutive particles are
 // interleaved in chunks of LIST_C entries: entry k of particle i lives at
 //   ((i / 64) * (cap / LIST_C) + k / LIST_C) * 64 * LIST_C + (i % 64) * LIST_C + k % LIST_C.
@@ -72,6 +82,16 @@ __device__ __host__ __forceinline__ size_t list_at(int i, int k, int cap)
 {
     return ((size_t)(i / WAVE) * (size_t)(cap / LIST_C) + (size_t)(k / LIST_C)) * (WAVE * LIST_C) +
            (size_t)(i % WAVE) * LIST_C + (size_t)(k % LIST_C);
+}
+
+// list_at(i, k, cap) == list_base(i, cap) + list_off(k) for 0 <= k < cap.
+__device__ __host__ __forceinline__ size_t list_base(int i, int cap)
+{
+    return (size_t)(i / WAVE) * (size_t)(cap / LIST_C) * (WAVE * LIST_C) + (size_t)(i % WAVE) * LIST_C;
+}
+__device__ __host__ __forceinline__ unsigned list_off(int k)
+{
+    return ((unsigned)k / LIST_C) * (WAVE * LIST_C) + (unsigned)k % LIST_C;
 }
 
 // Slot of entry kk of a list with `front` entries at the start and the rest

@@ -100,7 +100,7 @@ template <int D, class T> struct GravityVisitor {
     }
 };
 
-template <int D, class T> __global__ void __launch_bounds__(BLOCK) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     const bool valid = i < n;

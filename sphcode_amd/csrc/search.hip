@@ -85,15 +85,14 @@ __device__ __forceinline__ double min_image1(const BoxDev &b, int k, double d)
 }
 
 template <int D, bool IJ>
-__global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDev box, const int2 *groups, const int *num_groups,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_SEARCH))) search_kernel(State s, TreeDev t, BoxDev box, const int2 *groups, const int *num_groups,
                                                       SearchArgs a)
 {
     __shared__ int l_begin[SW][LEAFCAP];
     __shared__ int l_pre[SW][LEAFCAP + 1];
     __shared__ double l_ks[SW][LEAFCAP];
     __shared__ double c_x[SW][3][WAVE];
-    __shared__ double c_ks[SW][WAVE];
-    __shared__ double c_h[SW][WAVE];  // the candidate's own radius (split mode)
+    __shared__ double c_lim[SW][WAVE];  // IJ: the candidate's radius squared
     __shared__ int c_j[SW][WAVE];
 
     const int w = uniform(threadIdx.x / WAVE);
@@ -125,6 +124,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
         }
     }
     const double h2_own = h * h;
+    const size_t lbase = list_base(valid ? i : 0, a.cap);
 
     // group box: centre, half extent, largest radius (wave-uniform)
     double gc[D], ge[D];
@@ -132,10 +132,10 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
     for(int k = 0; k < D; ++k) {
         const double lo = wave_minv(valid ? xi[k] : 1.7976931348623157e308);
         const double hi = wave_maxv(valid ? xi[k] : -1.7976931348623157e308);
-        gc[k] = 0.5 * (lo + hi);
-        ge[k] = 0.5 * (hi - lo) * (1.0 + 1e-12) + 1e-300;
+        gc[k] = uniform_d(0.5 * (lo + hi));
+        ge[k] = uniform_d(0.5 * (hi - lo) * (1.0 + 1e-12) + 1e-300);
     }
-    const double hmax = wave_maxv(valid ? h : 0.0);
+    const double hmax = uniform_d(wave_maxv(valid ? h : 0.0));
 
     int *out = a.list;
     double *outr = a.list_r;
@@ -187,15 +187,8 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                 closed_skip = overlap && !leaf ? 0 : skip;
                 leafhit = overlap && leaf;
             }
-            // inclusive prefix maximum of closed_skip over the lanes
-            int incl = closed_skip;
-#pragma unroll
-            for(int off = 1; off < WAVE; off <<= 1) {
-                const int y = __shfl_up(incl, off, WAVE);
-                if(lane >= off) incl = incl > y ? incl : y;
-            }
-            int excl = __shfl_up(incl, 1, WAVE);
-            if(lane == 0) excl = 0;
+            const int incl = wave_incl_max_nonneg(closed_skip);
+            const int excl = wave_shr1(incl);
             const bool vis = in && v >= excl;
             visited += __popcll(__ballot(vis));
             const bool hit = vis && leafhit;
@@ -207,7 +200,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                 l_ks[w][at] = ks;
             }
             nleaf += uniform(__popcll(hits));
-            const int far = uniform(__shfl(incl, WAVE - 1, WAVE));
+            const int far = __builtin_amdgcn_readlane(incl, WAVE - 1);
             node = node + WAVE > far ? node + WAVE : far;
         }
         wave_lds_sync();
@@ -225,21 +218,26 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
         wave_lds_sync();
 
         // ---- phase 2: candidate tiles of 64
+        int lc = 0;  // last leaf starting at or before tb
         for(int tb = 0; tb < ncand; tb += WAVE) {
             // stage: each lane fetches one candidate and keeps it only if it
             // lies within the group's largest radius of the group box (the
             // box contains every x_i, so each lane's own test implies this)
             const int f = tb + lane;
+            // the candidate's leaf: the last leaf starting at or before f,
+            // found among the few leaves that start inside the tile
+            int lo = lc, nx = lc;
+            for(int L = lc + 1; L < nleaf; ++L) {
+                const int st = uniform(l_pre[w][L]);
+                if(st > tb + WAVE) break;
+                nx = L;
+                lo = f >= st ? L : lo;
+            }
+            lc = nx;
             bool keep = false;
             int j = 0;
             double xj[D], ksj = 0.0;
             if(f < ncand) {
-                int lo = 0, hi = nleaf - 1;  // last leaf with l_pre <= f
-                while(lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if(l_pre[w][mid] <= f) lo = mid;
-                    else hi = mid - 1;
-                }
                 j = l_begin[w][lo] + (f - l_pre[w][lo]);
                 double d2b = 0.0;
 #pragma unroll
@@ -262,8 +260,12 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
 #pragma unroll
                 for(int k = 0; k < D; ++k) c_x[w][k][pos] = xj[k];
                 if constexpr(IJ) {
-                    c_ks[w][pos] = ksj;
-                    if(a.split) c_h[w][pos] = s.f[F_SML][j];
+                    // split mode: r < max(sml_i, sml_j) (each radius bounds
+                    // that particle's final h unless truncated); otherwise
+                    // the reference's r < max(h_i, leaf's largest h).  As
+                    // squares: max(a, b)^2 == max(a^2, b^2) for a, b >= 0.
+                    const double kl = a.split ? s.f[F_SML][j] : ksj;
+                    c_lim[w][pos] = kl * kl;
                 }
             }
             const int m = uniform(__popcll(kept));
@@ -272,23 +274,23 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
             if(valid) {
                 // test every staged candidate (uniform loop, broadcast LDS
                 // reads), then append this lane's hits in staging order
-                uint64_t mine = 0;
-                for(int c = 0; c < m; ++c) {
+                auto hit = [&](int c) {
                     double xj[D], d[D];
 #pragma unroll
                     for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
                     const double r2 = sep2<D>(box, xi, xj, d);
                     double lim = h2_own;
                     if constexpr(IJ) {
-                        // split mode: r < max(sml_i, sml_j) (each radius
-                        // bounds that particle's final h unless truncated);
-                        // otherwise the reference's r < max(h_i, leaf's h)
-                        const double kl = a.split ? c_h[w][c] : c_ks[w][c];
-                        const double Hc = h < kl ? kl : h;
-                        lim = Hc * Hc;
+                        const double kl2 = c_lim[w][c];
+                        lim = kl2 > lim ? kl2 : lim;
                     }
-                    mine |= r2 < lim ? 1ull << c : 0ull;
-                }
+                    return r2 < lim;
+                };
+                uint32_t mlo = 0, mhi = 0;
+                const int m1 = m < 32 ? m : 32;
+                for(int c = 0; c < m1; ++c) mlo |= hit(c) ? 1u << c : 0u;
+                for(int c = 32; c < m; ++c) mhi |= hit(c) ? 1u << (c - 32) : 0u;
+                uint64_t mine = ((uint64_t)mhi << 32) | mlo;
                 while(mine) {
                     const int c = __builtin_ctzll(mine);
                     mine &= mine - 1ull;
@@ -300,7 +302,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                     const bool room = wf + wb < a.cap;
                     if(!a.split || r2 < h2_own) {
                         if(room) {
-                            const size_t at = list_at(i, wf, a.cap);
+                            const size_t at = lbase + list_off(wf);
                             out[at] = jj;
                             if(outr) outr[at] = sqrt(r2);
                             if(outrf) outrf[at] = (float)sqrt(r2);
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(BLOCK) search_kernel(State s, TreeDev t, BoxDe
                         ++count;
                     } else {
                         if(room) {
-                            out[list_at(i, a.cap - 1 - wb, a.cap)] = jj;
+                            out[lbase + list_off(a.cap - 1 - wb)] = jj;
                             ++wb;
                         }
                         ++count2;
