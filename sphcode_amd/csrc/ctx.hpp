@@ -48,6 +48,8 @@ template <class T> __device__ __forceinline__ T fld(const State &s, int f, int i
 template <class T> struct Acc {
     double s = 0.0;
     __device__ __forceinline__ void add(T x) { s += (double)x; }
+    // s += x * c with the FP64 factor x kept: one FP64 fused multiply-add
+    __device__ __forceinline__ void add_prod(double x, T c) { s = fma(x, (double)c, s); }
     __device__ __forceinline__ double val() const { return s; }
 };
 
@@ -59,6 +61,28 @@ struct TreeDev {
     int4 *rng;       // first particle, end particle, skip (pre-order successor
                      // after the subtree), level | leaf << 8
     int *num_nodes;  // device scalar
+};
+
+// Gravity walk records, rebuilt before every gravity pass so that one
+// scalar load brings all a wave needs at a node or at a leaf particle.
+struct __align__(64) GravNode {
+    double cx, cy, cz;   // centre of mass (zero at the root, as BHTree keeps it)
+    double gm;           // G * mass
+    double edge2;        // edge^2, for the opening test edge^2 > theta^2 d^2
+    int begin, end;      // particle range
+    int skip;            // pre-order successor after the subtree
+    int leaf;
+    double pad;
+};
+static_assert(sizeof(GravNode) == 64, "GravNode is one 64-byte record");
+template <class T> struct GravPart;
+template <> struct __align__(32) GravPart<float> {  // mixed: FP32 mirrors of m and 2/h
+    double x[3];
+    float m, einv;
+};
+template <> struct __align__(16) GravPart<double> {
+    double x[3];
+    double m, einv, pad;
 };
 
 // Register budgets, as the minimum number of waves per SIMD the compiler
@@ -138,6 +162,8 @@ struct Ctx {
     int *lvl = nullptr;        // per-level node counts, offsets, fill (device)
     int *lvl_host = nullptr;   // per-level counts and offsets (pinned host)
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
+    GravNode *gnode = nullptr; // gravity node records (gravity on)
+    void *gpart = nullptr;     // GravPart<T> per particle (gravity on)
     // search groups: particle ranges of at most 64 particles, each inside
     // the deepest cell holding at most 64 particles
     int2 *groups = nullptr;

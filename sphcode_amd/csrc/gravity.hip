@@ -21,54 +21,94 @@ struct GravArgs {
     double G;
     double theta2;
     int64_t *counters;
+    const GravNode *gnode;
+    const void *gpart;
 };
 
+// Node records: centre of mass, G m and edge^2 from the tree's moments and
+// geometry, the particle range, the successor and the leaf flag.
+__global__ void __launch_bounds__(BLOCK) pack_nodes_kernel(TreeDev t, GravNode *out, double G, int64_t cap)
+{
+    const int total = *t.num_nodes;
+    for(int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < total && v < cap; v += (int64_t)gridDim.x * BLOCK) {
+        const double4 g = t.geo[v];
+        const double4 m = t.mom[v];
+        const int4 r = t.rng[v];
+        GravNode o;
+        o.cx = m.x;
+        o.cy = m.y;
+        o.cz = m.z;
+        o.gm = G * m.w;
+        o.edge2 = g.w * g.w;
+        o.begin = r.x;
+        o.end = r.y;
+        o.skip = r.z > (int)v ? r.z : (int)v + 1;
+        o.leaf = (r.w & 256) != 0;
+        o.pad = 0.0;
+        out[v] = o;
+    }
+}
+
+// Particle records in tree order: position, mass and 2/h in the pair type
+// (FP32 mirrors in mixed mode, as the pair kernels read them).
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_parts_kernel(State s, int n, GravPart<T> *out)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i > n) return;
+    GravPart<T> p{};
+    if(i < n) {
+#pragma unroll
+        for(int k = 0; k < 3; ++k) p.x[k] = k < D ? s.f[F_X + k][i] : 0.0;
+        p.m = fld<T>(s, F_MASS, i);
+        p.einv = T(2) * fld<T>(s, F_HINV, i);
+    }
+    out[i] = p;  // i == n: the spare record
+}
+
 template <int D, class T> struct GravityVisitor {
-    const TreeDev &t;
     const BoxDev &box;
-    const State &s;
+    const GravPart<T> *gp;
     double G, theta2;
     double xi[D];
     T einv_i;   // 2 / h_i
     Acc<T> phi;
     Acc<T> acc[D];
 
-    __device__ __forceinline__ GravityVisitor(const TreeDev &t_, const BoxDev &b_, const State &s_) : t(t_), box(b_), s(s_) {}
+    __device__ __forceinline__ GravityVisitor(const BoxDev &b_, const GravPart<T> *gp_) : box(b_), gp(gp_) {}
 
     // Opening test edge^2 > theta^2 |x_i - com|^2 (FP64, all lanes); a lane
     // of `me` that does not open adds the monopole, the others add zero.
-    __device__ __forceinline__ bool node(int n, bool me)
+    __device__ __forceinline__ bool node(const GravNode &g, bool me)
     {
-        const double4 g = t.geo[n];
-        const double4 m = t.mom[n];
         double d[D];
-        d[0] = xi[0] - m.x;
-        if constexpr(D > 1) d[1] = xi[1] - m.y;
-        if constexpr(D > 2) d[2] = xi[2] - m.z;
+        d[0] = xi[0] - g.cx;
+        if constexpr(D > 1) d[1] = xi[1] - g.cy;
+        if constexpr(D > 2) d[2] = xi[2] - g.cz;
         min_image<D>(box, d);
         double d2 = d[0] * d[0];
 #pragma unroll
         for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
-        const bool o = g.w * g.w > theta2 * d2;
+        const bool o = g.edge2 > theta2 * d2;
         const bool far = me && !o;
-        const T gm = far ? (T)(G * m.w) : T(0);
+        const T gm = far ? (T)g.gm : T(0);
         const T rinv = rsqrt_t<T>(far ? (T)d2 : T(1));
         phi.add(-(gm * rinv));
         const T c = gm * rinv * rinv * rinv;
 #pragma unroll
-        for(int k = 0; k < D; ++k) acc[k].add(-((T)d[k] * c));
+        for(int k = 0; k < D; ++k) acc[k].add_prod(d[k], -c);
         return me && o;
     }
 
-    __device__ __forceinline__ void leaf(int4 r, bool open)
+    __device__ __forceinline__ void leaf(int b, int e, bool open)
     {
         const T w = open ? T(1) : T(0);
-        for(int j = r.x; j < r.y; ++j) {
+        for(int j = b; j < e; ++j) {
+            const GravPart<T> p = gp[j];
             double xj[D];
 #pragma unroll
-            for(int k = 0; k < D; ++k) xj[k] = s.f[F_X + k][j];
-            const T mj = fld<T>(s, F_MASS, j);
-            const T einv_j = T(2) * fld<T>(s, F_HINV, j);
+            for(int k = 0; k < D; ++k) xj[k] = p.x[k];
+            const T mj = p.m;
+            const T einv_j = p.einv;
             double dd[D];
             const double r2d = sep2<D>(box, xi, xj, dd);
             T rij[D];
@@ -95,7 +135,7 @@ template <int D, class T> struct GravityVisitor {
             phi.add(-(gm * pot));
             const T c = gm * acc3;
 #pragma unroll
-            for(int k = 0; k < D; ++k) acc[k].add(-(rij[k] * c));
+            for(int k = 0; k < D; ++k) acc[k].add_prod(dd[k], -c);
         }
     }
 };
@@ -104,7 +144,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     const bool valid = i < n;
-    GravityVisitor<D, T> v(t, box, s);
+    GravityVisitor<D, T> v(box, (const GravPart<T> *)a.gpart);
     v.G = a.G;
     v.theta2 = a.theta2;
     v.einv_i = T(0);
@@ -119,23 +159,33 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
-    const WalkCount wc = tree_walk(t, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+    const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
         for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k].val();
     }
     if(wave_lane() == 0) {
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)wc.pairs);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)wc.accepted);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)wc.opened);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)wc.visits);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)wc.pairs);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)wc.accepted);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)wc.opened);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)(unsigned)wc.visits);
     }
 }
 
-template <class T> static void gravity_t(Ctx &c, const GravArgs &a, const BoxDev &box)
+template <class T> static void gravity_t(Ctx &c, GravArgs a, const BoxDev &box)
 {
     const int n = (int)c.n;
+    a.gnode = c.gnode;
+    a.gpart = c.gpart;
+    hipLaunchKernelGGL(pack_nodes_kernel, dim3(std::min<int64_t>(grid_for(c.node_cap), 2048)), dim3(BLOCK), 0, c.stream, c.t,
+                       c.gnode, a.G, c.node_cap);
+    GravPart<T> *gp = (GravPart<T> *)c.gpart;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
+    case 2: hipLaunchKernelGGL((pack_parts_kernel<2, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
+    default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
+    }
     switch(c.D) {
     case 1: hipLaunchKernelGGL((gravity_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
     case 2: hipLaunchKernelGGL((gravity_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
@@ -146,6 +196,7 @@ template <class T> static void gravity_t(Ctx &c, const GravArgs &a, const BoxDev
 int gravity_compute(Ctx &c)
 {
     if(!c.p.gravity_is_valid) return SPHGPU_OK;
+    if(!c.gnode || !c.gpart) return set_error(SPHGPU_E_STATE, "gravity: no node records");
     GravArgs a{};
     a.G = c.p.gravity_constant;
     a.theta2 = c.p.gravity_theta * c.p.gravity_theta;

@@ -470,6 +470,11 @@ int tree_alloc(Ctx &c)
     SPHGPU_HIP(hipMalloc(&c.nchild, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.lvl_nodes, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.msum, c.node_cap * sizeof(double4)));
+    if(c.p.gravity_is_valid) {
+        SPHGPU_HIP(hipMalloc(&c.gnode, c.node_cap * sizeof(GravNode)));
+        // one spare record: a scalar load may fetch a whole record past the last
+        SPHGPU_HIP(hipMalloc(&c.gpart, (n + 1) * sizeof(GravPart<double>)));
+    }
     if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
     if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, 2 * NLEVEL * sizeof(int), hipHostMallocDefault));
     for(int k = 0; k < 2 * NLEVEL; ++k) c.lvl_host[k] = 0;

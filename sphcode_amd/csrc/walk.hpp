@@ -24,21 +24,23 @@ constexpr int WALK_DONE = 0x7fffffff;
 constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 
 // Visitor interface (every call is made by the whole wave):
-//   bool node(int node, bool me)         the lane's own decision at the node,
-//                                        for lanes with `me`; the visitor
-//                                        sums a node accepted without opening
-//                                        itself, masked rather than branched
-//   void leaf(int4 rng, bool open)       particles of an opened leaf, for the
-//                                        lanes with `open`
+//   bool node(const GravNode &g, bool me) the lane's own decision at the
+//                                        node, for lanes with `me`; the
+//                                        visitor sums a node accepted without
+//                                        opening itself, masked rather than
+//                                        branched
+//   void leaf(int b, int e, bool open)   particles [b, e) of an opened leaf,
+//                                        for the lanes with `open`
 // The walk state (cursor, masks, parked stack) is wave-uniform and kept in
-// scalar registers.  WalkCount returns, per wave, the node visits and the
+// scalar registers; each node is one 64-byte record (GravNode) read with
+// one scalar load.  WalkCount returns, per wave, the node visits and the
 // lane totals of opened nodes, accepted nodes and particle interactions.
 struct WalkCount {
-    int64_t visits = 0, opened = 0, accepted = 0, pairs = 0;
+    int visits = 0, opened = 0, accepted = 0, pairs = 0;
 };
 
 template <class V>
-__device__ __forceinline__ WalkCount tree_walk(const TreeDev &t, int total, bool valid, V &v, int *stk_node,
+__device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bool valid, V &v, int *stk_node,
                                                uint64_t *stk_mask)
 {
     WalkCount wc;
@@ -48,52 +50,50 @@ __device__ __forceinline__ WalkCount tree_walk(const TreeDev &t, int total, bool
     int sp = 0;                    // entries below the top, in LDS
     int top_node = WALK_DONE;      // top entry
     uint64_t top_mask = 0;
+    auto pop = [&]() {
+        if(sp > 0) {
+            --sp;
+            top_node = uniform(stk_node[sp]);
+            top_mask = uniform64(stk_mask[sp]);
+        } else {
+            top_node = WALK_DONE;
+            top_mask = 0;
+        }
+    };
     while(true) {
-        active = uniform64(active);
-        node = uniform(node);
-        top_node = uniform(top_node);
-        top_mask = uniform64(top_mask);
-        sp = uniform(sp);
-        if(active == 0 || node >= total) {
-            if(top_node == WALK_DONE) break;
-            node = top_node;
-            active = top_mask;
-            if(sp > 0) {
-                --sp;
-                top_node = stk_node[sp];
-                top_mask = stk_mask[sp];
+        // settle the cursor: resume parked lanes until some lane is live
+        // at a node (the visitor's sums are untouched here)
+        bool done = false;
+        while(true) {
+            if(active == 0 || node >= total) {
+                if(top_node == WALK_DONE) {
+                    done = true;
+                    break;
+                }
+                node = top_node;
+                active = top_mask;
+                pop();
+            } else if(top_node == node) {  // parked lanes rejoin
+                active |= top_mask;
+                pop();
             } else {
-                top_node = WALK_DONE;
-                top_mask = 0;
+                break;
             }
-            continue;
         }
-        if(top_node == node) {  // parked lanes rejoin
-            active |= top_mask;
-            if(sp > 0) {
-                --sp;
-                top_node = stk_node[sp];
-                top_mask = stk_mask[sp];
-            } else {
-                top_node = WALK_DONE;
-                top_mask = 0;
-            }
-            continue;
-        }
+        if(done) break;
         ++wc.visits;
-        const int4 r = t.rng[node];
-        const bool leaf = (r.w & 256) != 0;
-        const int skip = r.z > node ? r.z : node + 1;
+        const GravNode g = gn[node];
+        const int skip = g.skip;
         const bool me = (active >> lane) & 1ull;
-        const bool open = v.node(node, me);
+        const bool open = v.node(g, me);
         const uint64_t om = uniform64(__ballot(open));
         const int nopen = __popcll(om);
         wc.opened += nopen;
         wc.accepted += __popcll(active) - nopen;
-        if(leaf) {
+        if(g.leaf) {
             if(om) {
-                v.leaf(r, open);
-                wc.pairs += (int64_t)nopen * (r.y - r.x);
+                v.leaf(g.begin, g.end, open);
+                wc.pairs += nopen * (g.end - g.begin);
             }
             node = skip;
         } else if(om) {
