@@ -32,6 +32,11 @@ __global__ void fill_int_kernel(int *p, int64_t n, int v)
     if(i < n) p[i] = v;
 }
 
+void reset_work(Ctx &c, int slot)
+{
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.work + NUM_XCD * slot, (int64_t)NUM_XCD, 0);
+}
+
 __global__ void iota_kernel(int *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -275,10 +280,13 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     int dev = 0;
     (void)hipGetDevice(&dev);
     c->device = dev;
+    int cus = 0;
+    if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) c->num_cu = cus;
     hipError_t e;
     if((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
        (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
        (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
+       (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
        (e = hipHostMalloc(&c->counters_host, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
        (e = hipHostMalloc(&c->scal_host, NSCALARS * sizeof(double), hipHostMallocDefault)) != hipSuccess) {
         delete c;
@@ -316,6 +324,7 @@ int sphgpu_destroy(void *ptr)
     free_buffers(*c);
     if(c->scal) (void)hipFree(c->scal);
     if(c->counters) (void)hipFree(c->counters);
+    if(c->work) (void)hipFree(c->work);
     if(c->counters_host) (void)hipHostFree(c->counters_host);
     if(c->scal_host) (void)hipHostFree(c->scal_host);
     for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);

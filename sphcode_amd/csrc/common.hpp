@@ -56,6 +56,30 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 }
 __device__ __forceinline__ int xcd_block() { return uniform(xcd_block((int)blockIdx.x, (int)gridDim.x)); }
 
+// Persistent waves: tasks [0, ntask) are split into NUM_XCD contiguous
+// shares, and the waves of each XCD take the tasks of its share one at a
+// time from that XCD's counter (hardware deals block b to XCD b mod 8).  A
+// wave's next task is -1 when its share is used up.  Call with all lanes
+// active.
+struct TaskShare {
+    int *ctr;
+    int lo, hi;
+    __device__ __forceinline__ TaskShare(int *work, int ntask)
+    {
+        const int x = (int)(blockIdx.x % NUM_XCD);
+        ctr = work + x;
+        lo = (int)((int64_t)ntask * x / NUM_XCD);
+        hi = (int)((int64_t)ntask * (x + 1) / NUM_XCD);
+    }
+    __device__ __forceinline__ int next()
+    {
+        int t = 0;
+        if(wave_lane() == 0) t = atomicAdd(ctr, 1);
+        t = lo + __builtin_amdgcn_readfirstlane(t);
+        return t < hi ? t : -1;
+    }
+};
+
 template <class T> __device__ __forceinline__ T wave_sum(T v)
 {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

@@ -138,6 +138,8 @@ struct Ctx {
     int D = 3;
     hipStream_t stream = nullptr;
     int device = 0;
+    int num_cu = 256;         // compute units of the device
+    int *work = nullptr;      // per-XCD task counters of persistent kernels
 
     int64_t n = 0;        // particles
     int64_t cap_n = 0;    // allocated particles
@@ -250,6 +252,10 @@ int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV swit
 int force_compute(Ctx &c);
 // gravity.hip
 int gravity_compute(Ctx &c);
+// Zero the NUM_XCD task counters at c.work + NUM_XCD * slot before a
+// persistent launch (slot 0: the neighbour search).
+void reset_work(Ctx &c, int slot);
+constexpr int WORK_SLOTS = 1;
 // step.hip
 int step_init_fields(Ctx &c);
 int step_timestep(Ctx &c);       // dt into scal[S_DT]

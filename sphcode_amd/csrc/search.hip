@@ -46,6 +46,7 @@ struct SearchArgs {
     double ratio;
     double nngb;
     int64_t *counters;
+    int *work;           // per-XCD task counters, zeroed before the launch
 };
 
 constexpr int SW = BLOCK / WAVE;  // waves per block
@@ -98,13 +99,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     const int w = uniform(threadIdx.x / WAVE);
     const int lane = wave_lane();
     const int ngroups = uniform(*num_groups);
-    const int nwaves = gridDim.x * SW;
-    // each wave takes a contiguous run of groups; with the XCD-aware block
-    // order every XCD sweeps one contiguous stretch of the Morton curve
-    const int chunk = (ngroups + nwaves - 1) / nwaves;
-    const int g0 = (xcd_block() * SW + w) * chunk;
-    const int g1 = g0 + chunk < ngroups ? g0 + chunk : ngroups;
-    for(int grp = g0; grp < g1; ++grp) {
+    // persistent waves: groups are taken one at a time from this XCD's
+    // contiguous share of the Morton curve
+    TaskShare share(a.work, ngroups);
+    for(int grp = share.next(); grp >= 0; grp = share.next()) {
     const int2 gr = make_int2(uniform(groups[grp].x), uniform(groups[grp].y));
     const int i = gr.x + lane;
     const bool valid = i < gr.y;
@@ -345,8 +343,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
 template <int D, bool IJ> static void launch_search(Ctx &c, const SearchArgs &a)
 {
     const BoxDev box = make_box(c);
-    const int blocks = std::min(grid_for(c.n), 4096);
-    hipLaunchKernelGGL((search_kernel<D, IJ>), dim3(blocks), dim3(BLOCK), 0, c.stream, c.s, c.t, box, c.groups, c.num_groups, a);
+    // enough blocks to fill every CU (at most 8 blocks of 4 waves each), a
+    // multiple of the XCD count; blocks beyond that find no task and leave
+    const int blocks = std::max(NUM_XCD, std::min(grid_for(c.n, WAVE) / SW, 8 * c.num_cu) / NUM_XCD * NUM_XCD);
+    reset_work(c, 0);
+    SearchArgs aw = a;
+    aw.work = c.work;
+    hipLaunchKernelGGL((search_kernel<D, IJ>), dim3(blocks), dim3(BLOCK), 0, c.stream, c.s, c.t, box, c.groups, c.num_groups, aw);
 }
 
 template <bool IJ> static int search_dispatch(Ctx &c, const SearchArgs &a)
