@@ -81,7 +81,7 @@ static void free_buffers(Ctx &c)
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
     fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
-    fr(c.gnode); fr(c.gpart);
+    fr(c.gnode); fr(c.gpart); fr(c.posrec); fr(c.pair_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
@@ -102,6 +102,11 @@ static int alloc_buffers(Ctx &c, int64_t n)
         SPHGPU_HIP(hipMalloc(&c.alt.f[f], np * sizeof(double)));
         if(mixed && f >= F_V) SPHGPU_HIP(hipMalloc(&c.s.g[f], np * sizeof(float)));
     }
+    SPHGPU_HIP(hipMalloc(&c.posrec, np * sizeof(double4)));
+    // one buffer for the records of either pair pass (ForceRec is the larger)
+    static_assert(sizeof(ForceRec<float>) >= sizeof(PreRec<float>) && sizeof(ForceRec<double>) >= sizeof(PreRec<double>),
+                  "pair_rec holds either record");
+    SPHGPU_HIP(hipMalloc(&c.pair_rec, np * (mixed ? sizeof(ForceRec<float>) : sizeof(ForceRec<double>))));
     SPHGPU_HIP(hipMalloc(&c.s.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.alt.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.s.nb, np * sizeof(int)));

@@ -85,6 +85,31 @@ template <> struct __align__(16) GravPart<double> {
     double m, einv, pad;
 };
 
+// Neighbour records of the force pass, packed once per pass so that a pair
+// reads one FP64 (x, y, z, h) record for the cut and one record of the pair
+// fields (FP32 mirrors in mixed mode) instead of one load per field.
+template <class T> struct __align__(16 * sizeof(T)) ForceRec {
+    T h, hinv, m, rho, p, c, alpha, bal, u, gradh;
+    T v[3];
+    T pad[3];
+};
+static_assert(sizeof(ForceRec<float>) == 64 && sizeof(ForceRec<double>) == 128, "ForceRec size");
+
+// Neighbour records of the pre-interaction's final sweep: FP64 position,
+// and m, u, c, v in the pair type.
+template <class T> struct PreRec;
+template <> struct __align__(64) PreRec<float> {
+    double x[3];
+    float m, u, c, v[3];
+    float pad[4];
+};
+template <> struct __align__(32) PreRec<double> {
+    double x[3];
+    double m, u, c, v[3];
+    double pad[3];
+};
+static_assert(sizeof(PreRec<float>) == 64 && sizeof(PreRec<double>) == 96, "PreRec size");
+
 // Register budgets, as the minimum number of waves per SIMD the compiler
 // must fit (amdgpu_waves_per_eu), for kernels that wait on dependent
 // gathers and where more resident waves beat a few spilled registers
@@ -165,6 +190,9 @@ struct Ctx {
     int *lvl_host = nullptr;   // per-level counts and offsets (pinned host)
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
     GravNode *gnode = nullptr; // gravity node records (gravity on)
+    double4 *posrec = nullptr; // force pass: x, y, z, h per particle
+    void *pair_rec = nullptr;  // records of the running pair pass: PreRec<T>
+                               // (pre-interaction) or ForceRec<T> (force)
     void *gpart = nullptr;     // GravPart<T> per particle (gravity on)
     // search groups: particle ranges of at most 64 particles, each inside
     // the deepest cell holding at most 64 particles

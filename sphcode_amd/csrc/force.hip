@@ -26,7 +26,40 @@ struct ForceArgs {
     int64_t *counters;
     int *rec;             // recorded force sets (debug), same layout as list
     int *rec_cnt;
+    const double4 *posrec;  // x, y, z, h (FP64)
+    const void *frec;       // ForceRec<T>
 };
+
+// One pass over the particles: the records the pair loop reads for j, with
+// the fields the formulation reads (FP32 mirrors in mixed mode).
+template <int D, int TYPE, class T>
+__global__ void __launch_bounds__(BLOCK) pack_force_kernel(State s, int n, int use_u, double4 *pr, ForceRec<T> *fr)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    double4 p;
+    p.x = s.f[F_X][i];
+    p.y = D > 1 ? s.f[F_X + 1][i] : 0.0;
+    p.z = D > 2 ? s.f[F_X + 2][i] : 0.0;
+    p.w = s.f[F_SML][i];
+    pr[i] = p;
+    ForceRec<T> r{};
+    r.h = fld<T>(s, F_SML, i);
+    r.hinv = fld<T>(s, F_HINV, i);
+    r.m = fld<T>(s, F_MASS, i);
+    r.rho = fld<T>(s, F_DENS, i);
+    r.p = fld<T>(s, F_PRES, i);
+    r.c = fld<T>(s, F_SOUND, i);
+#pragma unroll
+    for(int l = 0; l < D; ++l) r.v[l] = fld<T>(s, F_V + l, i);
+    if constexpr(TYPE != SPHGPU_GSPH) {
+        r.alpha = fld<T>(s, F_ALPHA, i);
+        r.bal = fld<T>(s, F_BAL, i);
+        r.gradh = fld<T>(s, F_GRADH, i);
+    }
+    if(use_u) r.u = fld<T>(s, F_ENE, i);
+    fr[i] = r;
+}
 
 // van Leer limiter (src/gsph/g_fluid_force.cpp:27-36).
 template <class T> __device__ __forceinline__ T van_leer(T a, T b)
@@ -100,13 +133,16 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 
         Acc<T> acc[D], dene;
         int used = 0;
+        const ForceRec<T> *frec = (const ForceRec<T> *)a.frec;
         for(int kk = 0; kk < total; ++kk) {
             const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
+            const double4 pj4 = a.posrec[j];
             double xj[D], d[D];
-#pragma unroll
-            for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
+            xj[0] = pj4.x;
+            if constexpr(D > 1) xj[1] = pj4.y;
+            if constexpr(D > 2) xj[2] = pj4.z;
             const double r2 = sep2<D>(box, xi, xj, d);
-            const double hj_d = s.f[F_SML][j];
+            const double hj_d = pj4.w;
             const double hmax = hi_d < hj_d ? hj_d : hi_d;
             T r;
             if constexpr(FP64) {
@@ -121,7 +157,8 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             T rij[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) rij[l] = (T)d[l];
-            const Kern kj(fld<T>(s, F_SML, j), fld<T>(s, F_HINV, j));
+            const ForceRec<T> fj = frec[j];
+            const Kern kj(fj.h, fj.hinv);
             const T cwi = ki.dw_coef(ki.q_of(r), r);
             const T cwj = kj.dw_coef(kj.q_of(r), r);
             T dwi[D], dwj[D], vj[D];
@@ -129,12 +166,12 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             for(int l = 0; l < D; ++l) {
                 dwi[l] = rij[l] * cwi;
                 dwj[l] = rij[l] * cwj;
-                vj[l] = fld<T>(s, F_V + l, j);
+                vj[l] = fj.v[l];
             }
-            const T mj = fld<T>(s, F_MASS, j);
-            const T rhoj = fld<T>(s, F_DENS, j);
-            const T pj = fld<T>(s, F_PRES, j);
-            const T cj = fld<T>(s, F_SOUND, j);
+            const T mj = fj.m;
+            const T rhoj = fj.rho;
+            const T pj = fj.p;
+            const T cj = fj.c;
 
             if constexpr(TYPE == SPHGPU_GSPH) {
                 const T rinv = T(1) / r;
@@ -210,15 +247,15 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 const T vr = dotd<D, T>(vij, rij);
                 T pij = T(0);
                 if(vr < T(0)) {
-                    const T alpha = T(0.5) * (alphai + fld<T>(s, F_ALPHA, j));
-                    const T bal = T(0.5) * (bali + fld<T>(s, F_BAL, j));
+                    const T alpha = T(0.5) * (alphai + fj.alpha);
+                    const T bal = T(0.5) * (bali + fj.bal);
                     const T w = vr / r;
                     const T vsig = ci + cj - T(3) * w;
                     pij = T(-0.5) * bal * alpha * vsig * w * (T(2) / (rhoi + rhoj));
                 }
                 T ac = T(0);
                 if(a.use_ac) {
-                    const T uj = fld<T>(s, F_ENE, j);
+                    const T uj = fj.u;
                     const T vs = a.gravity ? fabs(vr / r) : sqrt(T(2) * fabs(pi_ - pj) / (rhoi + rhoj));
                     ac = (T)a.alpha_ac * mj * vs * (ui - uj) * dotd<D, T>(dwij, rij) / r;
                 }
@@ -226,16 +263,16 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 const T vdwij = dotd<D, T>(vij, dwij);
                 T ci_f, cj_f, de;
                 if constexpr(TYPE == SPHGPU_DISPH) {
-                    const T uj = fld<T>(s, F_ENE, j);
+                    const T uj = fj.u;
                     const T fij = T(1) - gradhi / (mj * uj);
-                    const T fji = T(1) - fld<T>(s, F_GRADH, j) * mu_inv_i;
+                    const T fji = T(1) - fj.gradh * mu_inv_i;
                     const T u_p_j = uj / pj;
                     ci_f = mj * (g2u_p_i * uj * fij + T(0.5) * pij);
                     cj_f = mj * (g2u_i * u_p_j * fji + T(0.5) * pij);
                     de = mj * g2u_p_i * uj * fij * vdwi;
                 } else {
                     ci_f = mj * (p_rho2_i * gradhi + T(0.5) * pij);
-                    cj_f = mj * (pj / (rhoj * rhoj) * fld<T>(s, F_GRADH, j) + T(0.5) * pij);
+                    cj_f = mj * (pj / (rhoj * rhoj) * fj.gradh + T(0.5) * pij);
                     de = mj * p_rho2_i * gradhi * vdwi;
                 }
 #pragma unroll
@@ -255,6 +292,9 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 
 template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, const ForceArgs &a, const BoxDev &box)
 {
+    const int use_u = TYPE == SPHGPU_DISPH || a.use_ac;
+    hipLaunchKernelGGL((pack_force_kernel<D, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, use_u,
+                       c.posrec, (ForceRec<T> *)c.pair_rec);
     hipLaunchKernelGGL((force_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
@@ -293,6 +333,8 @@ int force_compute(Ctx &c)
     a.counters = c.counters;
     a.rec = c.record ? c.rec_force : nullptr;
     a.rec_cnt = c.record ? c.rec_force_cnt : nullptr;
+    a.posrec = c.posrec;
+    a.frec = c.pair_rec;
     const BoxDev box = make_box(c);
     if(c.p.precision == SPHGPU_MIXED) force_t<float>(c, a, box);
     else force_t<double>(c, a, box);

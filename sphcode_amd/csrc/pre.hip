@@ -43,7 +43,24 @@ struct PreArgs {
     int second_order;
     int64_t *counters;
     double *partial;      // per-block min of h / v_sig
+    const void *prec;     // PreRec<T> per particle
 };
+
+// Records read for j by the final sweep (FP32 mirrors in mixed mode).
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_pre_kernel(State s, int n, PreRec<T> *out)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    PreRec<T> r{};
+#pragma unroll
+    for(int k = 0; k < 3; ++k) r.x[k] = k < D ? s.f[F_X + k][i] : 0.0;
+    r.m = fld<T>(s, F_MASS, i);
+    r.u = fld<T>(s, F_ENE, i);
+    r.c = fld<T>(s, F_SOUND, i);
+#pragma unroll
+    for(int k = 0; k < D; ++k) r.v[k] = fld<T>(s, F_V + k, i);
+    out[i] = r;
+}
 
 template <class T> __device__ __forceinline__ T list_r(const PreArgs &a, size_t k)
 {
@@ -166,17 +183,19 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
         const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
+        const PreRec<T> *prec = (const PreRec<T> *)a.prec;
         for(int k = 0; k < cnt; ++k) {
             const T r = list_r<T>(a, list_at(i, k, a.cap));
             if(!(r < ht)) continue;
             const int j = a.list[list_at(i, k, a.cap)];
             ++count;
-            const T mj = fld<T>(s, F_MASS, j);
+            const PreRec<T> pj = prec[j];
+            const T mj = pj.m;
             const T q = ker.q_of(r);
             const T w = ker.w(q);
             if constexpr(TYPE == SPHGPU_DISPH) {
                 const T dh = ker.dhw(q);
-                const T uj = fld<T>(s, F_ENE, j);
+                const T uj = pj.u;
                 rho.add(mj * w);
                 nd.add(w);
                 pb.add(mj * uj * w);
@@ -192,19 +211,19 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
             if(j != i || need_dv) {
                 double xj[D], d[D];
 #pragma unroll
-                for(int l = 0; l < D; ++l) xj[l] = s.f[F_X + l][j];
+                for(int l = 0; l < D; ++l) xj[l] = pj.x[l];
                 sep2<D>(box, xi, xj, d);
 #pragma unroll
                 for(int l = 0; l < D; ++l) {
                     rij[l] = (T)d[l];
-                    vij[l] = vi[l] - fld<T>(s, F_V + l, j);
+                    vij[l] = vi[l] - pj.v[l];
                 }
             }
             if(j != i) {
                 T rv = rij[0] * vij[0];
 #pragma unroll
                 for(int l = 1; l < D; ++l) rv += rij[l] * vij[l];
-                const T vs = ci_t + fld<T>(s, F_SOUND, j) - T(3) * rv / r;
+                const T vs = ci_t + pj.c - T(3) * rv / r;
                 vsig_max = vs > vsig_max ? vs : vsig_max;
             }
             if(need_dv) {
@@ -213,7 +232,7 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 #pragma unroll
                 for(int l = 0; l < D; ++l) dw[l] = rij[l] * cw;
                 if constexpr(TYPE == SPHGPU_GSPH) {
-                    const T du = mj * (fld<T>(s, F_ENE, j) - ui_t);
+                    const T du = mj * (pj.u - ui_t);
 #pragma unroll
                     for(int l = 0; l < D; ++l) {
                         gd[l].add(dw[l] * mj);
@@ -226,7 +245,7 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                         for(int l = 0; l < D; ++l) gv[c][l].add(dw[l] * dvc);
                     }
                 } else {
-                    const T wgt = TYPE == SPHGPU_DISPH ? mj * fld<T>(s, F_ENE, j) : mj;
+                    const T wgt = TYPE == SPHGPU_DISPH ? mj * pj.u : mj;
                     T vd = vij[0] * dw[0];
 #pragma unroll
                     for(int l = 1; l < D; ++l) vd += vij[l] * dw[l];
@@ -339,6 +358,7 @@ static PreArgs make_args(Ctx &c)
     a.second_order = c.p.gsph_is_2nd_order;
     a.counters = c.counters;
     a.partial = c.partial;
+    a.prec = c.pair_rec;
     return a;
 }
 
@@ -376,6 +396,8 @@ int pre_initial(Ctx &c)
 
 template <int D, int K, int TYPE, class T> static void launch_pre(Ctx &c, const PreArgs &a, const BoxDev &box)
 {
+    hipLaunchKernelGGL((pack_pre_kernel<D, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n,
+                       (PreRec<T> *)c.pair_rec);
     hipLaunchKernelGGL((pre_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
