@@ -60,27 +60,20 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
             top_mask = 0;
         }
     };
+    // The top of the parked stack is never below the cursor: a parked index
+    // is the skip of an ancestor of the cursor's node, and no move inside
+    // that subtree passes it.  So parked lanes rejoin exactly when the
+    // cursor reaches their index, and the walk ends at total (where lanes
+    // parked at total have rejoined first).  `active` only shrinks to the
+    // lanes opening a node, which are never none, so a wave with a valid
+    // lane keeps one.
+    if(active == 0) return wc;
     while(true) {
-        // settle the cursor: resume parked lanes until some lane is live
-        // at a node (the visitor's sums are untouched here)
-        bool done = false;
-        while(true) {
-            if(active == 0 || node >= total) {
-                if(top_node == WALK_DONE) {
-                    done = true;
-                    break;
-                }
-                node = top_node;
-                active = top_mask;
-                pop();
-            } else if(top_node == node) {  // parked lanes rejoin
-                active |= top_mask;
-                pop();
-            } else {
-                break;
-            }
+        while(node == top_node) {
+            active |= top_mask;
+            pop();
         }
-        if(done) break;
+        if(node >= total) break;
         ++wc.visits;
         const GravNode g = gn[node];
         const int skip = g.skip;
