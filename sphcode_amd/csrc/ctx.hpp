@@ -72,7 +72,8 @@ struct __align__(64) GravNode {
     int begin, end;      // particle range
     int skip;            // pre-order successor after the subtree
     int leaf;
-    double pad;
+    float gmf;           // G * mass rounded to FP32 (mixed mode)
+    int pad;
 };
 static_assert(sizeof(GravNode) == 64, "GravNode is one 64-byte record");
 template <class T> struct GravPart;

@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(BLOCK) pack_nodes_kernel(TreeDev t, GravNode *
         o.end = r.y;
         o.skip = r.z > (int)v ? r.z : (int)v + 1;
         o.leaf = (r.w & 256) != 0;
-        o.pad = 0.0;
+        o.gmf = (float)o.gm;
+        o.pad = 0;
         out[v] = o;
     }
 }
@@ -90,7 +91,10 @@ template <int D, class T> struct GravityVisitor {
         for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
         const bool o = g.edge2 > theta2 * d2;
         const bool far = me && !o;
-        const T gm = far ? (T)g.gm : T(0);
+        T gmn;
+        if constexpr(sizeof(T) == 4) gmn = g.gmf;
+        else gmn = g.gm;
+        const T gm = far ? gmn : T(0);
         const T rinv = rsqrt_t<T>(far ? (T)d2 : T(1));
         phi.add(-(gm * rinv));
         const T c = gm * rinv * rinv * rinv;
