@@ -110,7 +110,7 @@ template <int D> __global__ void __launch_bounds__(BLOCK) adj_seed_kernel(State 
 template <class T> __device__ __forceinline__ T pair_r(double r2)
 {
     if constexpr(sizeof(T) == 8) return sqrt(r2);
-    else return (float)sqrt(r2);
+    else return gather_r_f32(r2);
 }
 
 // pass 1: force pairs (src/fluid_force.cpp:26-106)

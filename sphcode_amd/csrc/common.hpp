@@ -129,6 +129,13 @@ __device__ __forceinline__ int wave_incl_max_nonneg(int x)
 // Lane l receives lane l-1's value; lane 0 receives 0 (DPP wave_shr:1).
 __device__ __forceinline__ int wave_shr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
 
+// |r| in FP32 of a gather candidate from its FP64 r^2 (mixed mode): the
+// hardware FP64 square root (one instruction, no refinement sequence)
+// rounded to FP32, which is the correctly rounded FP32 |r| except when the
+// FP64 result falls within its error of an FP32 rounding boundary.  The
+// search stores it and the adjoint recomputes it with this same function.
+__device__ __forceinline__ float gather_r_f32(double r2) { return (float)__builtin_amdgcn_sqrt(r2); }
+
 // --------------------------------------------------------------- scalar math
 template <class T> __host__ __device__ __forceinline__ T sq(T x) { return x * x; }
 template <class T> __host__ __device__ __forceinline__ T cube(T x) { return x * x * x; }

@@ -99,17 +99,15 @@ static_assert(sizeof(ForceRec<float>) == 64 && sizeof(ForceRec<double>) == 128, 
 // Neighbour records of the pre-interaction's final sweep: FP64 position,
 // and m, u, c, v in the pair type.
 template <class T> struct PreRec;
-template <> struct __align__(64) PreRec<float> {
+template <> struct __align__(16) PreRec<float> {
     double x[3];
     float m, u, c, v[3];
-    float pad[4];
 };
-template <> struct __align__(32) PreRec<double> {
+template <> struct __align__(16) PreRec<double> {
     double x[3];
     double m, u, c, v[3];
-    double pad[3];
 };
-static_assert(sizeof(PreRec<float>) == 64 && sizeof(PreRec<double>) == 96, "PreRec size");
+static_assert(sizeof(PreRec<float>) == 48 && sizeof(PreRec<double>) == 72 + 8, "PreRec size");
 
 // Register budgets, as the minimum number of waves per SIMD the compiler
 // must fit (amdgpu_waves_per_eu), for kernels that wait on dependent

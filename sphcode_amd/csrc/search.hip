@@ -303,9 +303,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
                             const size_t at = lbase + list_off(wf);
                             out[at] = jj;
                             if(outr) outr[at] = sqrt(r2);
-                            // mixed mode keeps |r| in FP32 only: the FP32
-                            // square root of the rounded r^2 (within 1 ulp)
-                            if(outrf) outrf[at] = sqrtf((float)r2);
+                            if(outrf) outrf[at] = gather_r_f32(r2);
                             ++wf;
                         }
                         ++count;
