@@ -89,12 +89,11 @@ template <> struct __align__(16) GravPart<double> {
 // Neighbour records of the force pass, packed once per pass so that a pair
 // reads one FP64 (x, y, z, h) record for the cut and one record of the pair
 // fields (FP32 mirrors in mixed mode) instead of one load per field.
-template <class T> struct __align__(16 * sizeof(T)) ForceRec {
-    T h, hinv, m, rho, p, c, alpha, bal, u, gradh;
+template <class T> struct __align__(16) ForceRec {
+    T hinv, m, rho, p, c, alpha, bal, u, gradh;  // the kernels need 1/h only
     T v[3];
-    T pad[3];
 };
-static_assert(sizeof(ForceRec<float>) == 64 && sizeof(ForceRec<double>) == 128, "ForceRec size");
+static_assert(sizeof(ForceRec<float>) == 48 && sizeof(ForceRec<double>) == 96, "ForceRec size");
 
 // Neighbour records of the pre-interaction's final sweep: FP64 position,
 // and m, u, c, v in the pair type.
@@ -107,7 +106,7 @@ template <> struct __align__(16) PreRec<double> {
     double x[3];
     double m, u, c, v[3];
 };
-static_assert(sizeof(PreRec<float>) == 48 && sizeof(PreRec<double>) == 72 + 8, "PreRec size");
+static_assert(sizeof(PreRec<float>) == 48 && sizeof(PreRec<double>) == 80, "PreRec size");
 
 // Register budgets, as the minimum number of waves per SIMD the compiler
 // must fit (amdgpu_waves_per_eu), for kernels that wait on dependent

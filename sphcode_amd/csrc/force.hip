@@ -44,7 +44,6 @@ __global__ void __launch_bounds__(BLOCK) pack_force_kernel(State s, int n, int u
     p.w = s.f[F_SML][i];
     pr[i] = p;
     ForceRec<T> r{};
-    r.h = fld<T>(s, F_SML, i);
     r.hinv = fld<T>(s, F_HINV, i);
     r.m = fld<T>(s, F_MASS, i);
     r.rho = fld<T>(s, F_DENS, i);
@@ -158,7 +157,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 #pragma unroll
             for(int l = 0; l < D; ++l) rij[l] = (T)d[l];
             const ForceRec<T> fj = frec[j];
-            const Kern kj(fj.h, fj.hinv);
+            const Kern kj(T(0), fj.hinv);  // the kernel object reads 1/h only
             const T cwi = ki.dw_coef(ki.q_of(r), r);
             const T cwj = kj.dw_coef(kj.q_of(r), r);
             T dwi[D], dwj[D], vj[D];
