@@ -81,7 +81,7 @@ static void free_buffers(Ctx &c)
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
     fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
-    fr(c.gnode); fr(c.gpart); fr(c.posrec); fr(c.pair_rec);
+    fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.posrec); fr(c.pair_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
@@ -289,6 +289,10 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) c->num_cu = cus;
     hipError_t e;
     if((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+       (e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming)) != hipSuccess ||
+       (e = hipEventCreate(&c->ev_g[0])) != hipSuccess || (e = hipEventCreate(&c->ev_g[1])) != hipSuccess ||
        (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
        (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
@@ -325,6 +329,7 @@ int sphgpu_destroy(void *ptr)
     Ctx *c = as_ctx(ptr);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if(c->gstream) (void)hipStreamSynchronize(c->gstream);
     adjoint_free(*c);
     free_buffers(*c);
     if(c->scal) (void)hipFree(c->scal);
@@ -334,6 +339,9 @@ int sphgpu_destroy(void *ptr)
     if(c->scal_host) (void)hipHostFree(c->scal_host);
     for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
     if(c->stream) (void)hipStreamDestroy(c->stream);
+    if(c->gstream) (void)hipStreamDestroy(c->gstream);
+    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_g[0], c->ev_g[1]})
+        if(e) (void)hipEventDestroy(e);
     delete c;
     return SPHGPU_OK;
 }
@@ -548,9 +556,24 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     mark(3);
     if((rc = stage_pre(c))) return rc;
     mark(4);
+    // gravity (vector-ALU bound) runs on the second stream while the force
+    // pass (bound by L2 misses on neighbour records) runs here; the gravity
+    // acceleration is added to acc once both are done
+    const bool grav = c.p.gravity_is_valid;
+    if(grav) {
+        SPHGPU_HIP(hipEventRecord(c.ev_pre, c.stream));
+        SPHGPU_HIP(hipStreamWaitEvent(c.gstream, c.ev_pre, 0));
+        if(c.timers_on) (void)hipEventRecord(c.ev_g[0], c.gstream);
+        if((rc = gravity_launch(c, c.gstream))) return rc;
+        if(c.timers_on) (void)hipEventRecord(c.ev_g[1], c.gstream);
+        SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
+    }
     if((rc = stage_force(c))) return rc;
     mark(5);
-    if((rc = gravity_compute(c))) return rc;
+    if(grav) {
+        SPHGPU_HIP(hipStreamWaitEvent(c.stream, c.ev_grav, 0));
+        if((rc = gravity_combine(c, c.stream))) return rc;
+    }
     mark(6);
     if((rc = step_correct(c))) return rc;
     mark(7);
@@ -563,7 +586,11 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     if(c.timers_on) {
         for(int k = 0; k < 7; ++k) {
             float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            // with gravity on, its timer is the walk's own span on its
+            // stream, which overlaps the force stage; the wait for its end
+            // and the combine (ev[5] to ev[6]) are in no stage
+            if(k == 5 && grav) (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
+            else (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
             c.timer_ms[k] += ms;
         }
     }

@@ -160,6 +160,10 @@ struct Ctx {
     sphgpu_params p{};
     int D = 3;
     hipStream_t stream = nullptr;
+    // second stream (default priority) on which sphgpu_integrate runs the
+    // gravity pass alongside the force pass
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[2] = {nullptr, nullptr};
     int device = 0;
     int num_cu = 256;         // compute units of the device
     int *work = nullptr;      // per-XCD task counters of persistent kernels
@@ -188,6 +192,7 @@ struct Ctx {
     int *lvl_host = nullptr;   // per-level counts and offsets (pinned host)
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
     GravNode *gnode = nullptr; // gravity node records (gravity on)
+    double *gacc = nullptr;    // gravity's acceleration, D x cap_n (gravity on)
     double4 *posrec = nullptr; // force pass: x, y, z, h per particle
     void *pair_rec = nullptr;  // records of the running pair pass: PreRec<T>
                                // (pre-interaction) or ForceRec<T> (force)
@@ -278,6 +283,11 @@ int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV swit
 int force_compute(Ctx &c);
 // gravity.hip
 int gravity_compute(Ctx &c);
+// The gravity pass in two parts: the walk on stream st writes phi and its
+// acceleration into gacc; the combine adds gacc to acc (after the force
+// pass has assigned acc).  gravity_compute runs both on c.stream.
+int gravity_launch(Ctx &c, hipStream_t st);
+int gravity_combine(Ctx &c, hipStream_t st);
 // Zero the NUM_XCD task counters at c.work + NUM_XCD * slot before a
 // persistent launch (slot 0: the neighbour search).
 void reset_work(Ctx &c, int slot);

@@ -5,8 +5,10 @@
 // A cell is opened when edge^2 > theta^2 |x_i - com|^2; an opened leaf is
 // summed particle by particle, i itself included (the self-potential), with
 // the Hernquist & Katz spline softening averaged over h_i and h_j; an
-// accepted cell adds its monopole.  phi is reset and acc is accumulated on
-// top of the hydrodynamic acceleration.  The walk is the wave-coherent
+// accepted cell adds its monopole.  phi is reset; the walk's acceleration
+// goes to its own buffer and is then added to the hydrodynamic
+// acceleration (gravity_combine), so the walk may run while the force pass
+// assigns acc.  The walk is the wave-coherent
 // pre-order walk of walk.hpp, so each lane keeps its own opening decisions.
 //
 // T is the interaction arithmetic.  The opening test always runs in FP64,
@@ -23,7 +25,18 @@ struct GravArgs {
     int64_t *counters;
     const GravNode *gnode;
     const void *gpart;
+    double *gacc;        // D x stride, the walk's acceleration
+    int64_t stride;
 };
+
+// acc += gravity's acceleration (after the force pass assigned acc)
+template <int D> __global__ void __launch_bounds__(BLOCK) gravity_combine_kernel(State s, int n, const double *gacc, int64_t stride)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if(i >= n) return;
+#pragma unroll
+    for(int k = 0; k < D; ++k) s.f[F_A + k][i] += gacc[k * stride + i];
+}
 
 // Node records: centre of mass, G m and edge^2 from the tree's moments and
 // geometry, the particle range, the successor and the leaf flag.
@@ -167,7 +180,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
-        for(int k = 0; k < D; ++k) s.f[F_A + k][i] += v.acc[k].val();
+        for(int k = 0; k < D; ++k) a.gacc[k * a.stride + i] = v.acc[k].val();
     }
     if(wave_lane() == 0) {
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)wc.pairs);
@@ -177,39 +190,60 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     }
 }
 
-template <class T> static void gravity_t(Ctx &c, GravArgs a, const BoxDev &box)
+template <class T> static void gravity_t(Ctx &c, GravArgs a, const BoxDev &box, hipStream_t st)
 {
     const int n = (int)c.n;
     a.gnode = c.gnode;
     a.gpart = c.gpart;
-    hipLaunchKernelGGL(pack_nodes_kernel, dim3(std::min<int64_t>(grid_for(c.node_cap), 2048)), dim3(BLOCK), 0, c.stream, c.t,
+    a.gacc = c.gacc;
+    a.stride = c.cap_n;
+    hipLaunchKernelGGL(pack_nodes_kernel, dim3(std::min<int64_t>(grid_for(c.node_cap), 2048)), dim3(BLOCK), 0, st, c.t,
                        c.gnode, a.G, c.node_cap);
     GravPart<T> *gp = (GravPart<T> *)c.gpart;
     switch(c.D) {
-    case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
-    case 2: hipLaunchKernelGGL((pack_parts_kernel<2, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
-    default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, c.stream, c.s, n, gp); break;
+    case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+    case 2: hipLaunchKernelGGL((pack_parts_kernel<2, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+    default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
     }
     switch(c.D) {
-    case 1: hipLaunchKernelGGL((gravity_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
-    case 2: hipLaunchKernelGGL((gravity_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
-    default: hipLaunchKernelGGL((gravity_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, c.t, box, n, a); break;
+    case 1: hipLaunchKernelGGL((gravity_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
     }
 }
 
-int gravity_compute(Ctx &c)
+int gravity_launch(Ctx &c, hipStream_t st)
 {
     if(!c.p.gravity_is_valid) return SPHGPU_OK;
-    if(!c.gnode || !c.gpart) return set_error(SPHGPU_E_STATE, "gravity: no node records");
+    if(!c.gnode || !c.gpart || !c.gacc) return set_error(SPHGPU_E_STATE, "gravity: no node records");
     GravArgs a{};
     a.G = c.p.gravity_constant;
     a.theta2 = c.p.gravity_theta * c.p.gravity_theta;
     a.counters = c.counters;
     const BoxDev box = make_box(c);
-    if(c.p.precision == SPHGPU_MIXED) gravity_t<float>(c, a, box);
-    else gravity_t<double>(c, a, box);
+    if(c.p.precision == SPHGPU_MIXED) gravity_t<float>(c, a, box, st);
+    else gravity_t<double>(c, a, box, st);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
+}
+
+int gravity_combine(Ctx &c, hipStream_t st)
+{
+    if(!c.p.gravity_is_valid) return SPHGPU_OK;
+    const int n = (int)c.n;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL(gravity_combine_kernel<1>, dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, n, c.gacc, c.cap_n); break;
+    case 2: hipLaunchKernelGGL(gravity_combine_kernel<2>, dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, n, c.gacc, c.cap_n); break;
+    default: hipLaunchKernelGGL(gravity_combine_kernel<3>, dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, n, c.gacc, c.cap_n); break;
+    }
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int gravity_compute(Ctx &c)
+{
+    int rc = gravity_launch(c, c.stream);
+    return rc ? rc : gravity_combine(c, c.stream);
 }
 
 }  // namespace sphgpu

@@ -474,6 +474,7 @@ int tree_alloc(Ctx &c)
         SPHGPU_HIP(hipMalloc(&c.gnode, c.node_cap * sizeof(GravNode)));
         // one spare record: a scalar load may fetch a whole record past the last
         SPHGPU_HIP(hipMalloc(&c.gpart, (n + 1) * sizeof(GravPart<double>)));
+        SPHGPU_HIP(hipMalloc(&c.gacc, (size_t)c.D * n * sizeof(double)));
     }
     if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
     if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, 2 * NLEVEL * sizeof(int), hipHostMallocDefault));
