@@ -568,7 +568,11 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
         if(c.timers_on) (void)hipEventRecord(c.ev_g[1], c.gstream);
         SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
     }
-    if((rc = stage_force(c))) return rc;
+    if((rc = stage_force(c))) {
+        // later work on the main stream must not overtake the walk
+        if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
+        return rc;
+    }
     mark(5);
     if(grav) {
         SPHGPU_HIP(hipStreamWaitEvent(c.stream, c.ev_grav, 0));
