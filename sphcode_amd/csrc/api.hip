@@ -81,7 +81,7 @@ static void free_buffers(Ctx &c)
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
     fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
-    fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.posrec); fr(c.pair_rec);
+    fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.leafhit); fr(c.leafcnt); fr(c.posrec); fr(c.pair_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
@@ -154,11 +154,27 @@ static int stage_tree(Ctx &c)
     return SPHGPU_OK;
 }
 
+// sphgpu_integrate's gravity walk on the second stream, ordered after the
+// work queued so far on the main stream.
+static int launch_gravity_walk(Ctx &c)
+{
+    SPHGPU_HIP(hipEventRecord(c.ev_pre, c.stream));
+    SPHGPU_HIP(hipStreamWaitEvent(c.gstream, c.ev_pre, 0));
+    if(c.timers_on) (void)hipEventRecord(c.ev_g[0], c.gstream);
+    int rc = gravity_walk(c, c.gstream);
+    if(c.timers_on) (void)hipEventRecord(c.ev_g[1], c.gstream);
+    SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
+    return rc;
+}
+
 // The pre-interaction's candidate search.  After the first call, one walk
 // finds both the gather candidates and the symmetric pairs the force stage
 // needs (search_combined); the first call keeps the reference's initial
-// smoothing and separate searches.
-static int stage_pre(Ctx &c)
+// smoothing and separate searches.  With walk_after_search the gravity
+// walk is started on the second stream once the search is queued, so that
+// it runs alongside the pre-interaction rather than the search (both the
+// walk and the search are bound by vector-ALU issue).
+static int stage_pre(Ctx &c, bool walk_after_search = false)
 {
     int rc;
     const bool combined = !c.first_pre;
@@ -179,6 +195,7 @@ static int stage_pre(Ctx &c)
         SPHGPU_HIP(hipMemcpyAsync(c.rec_gather_cnt, c.list_cnt, c.cap_n * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
         c.have_gather = true;
     }
+    if(walk_after_search && (rc = launch_gravity_walk(c))) return rc;
     if((rc = pre_compute(c))) return rc;
     c.lists_symmetric = false;
     if(combined) {
@@ -293,6 +310,7 @@ int sphgpu_create(const sphgpu_params *params, void **out)
        (e = hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming)) != hipSuccess ||
        (e = hipEventCreate(&c->ev_g[0])) != hipSuccess || (e = hipEventCreate(&c->ev_g[1])) != hipSuccess ||
+       (e = hipEventCreate(&c->ev_g[2])) != hipSuccess || (e = hipEventCreate(&c->ev_g[3])) != hipSuccess ||
        (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
        (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
@@ -340,7 +358,7 @@ int sphgpu_destroy(void *ptr)
     for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     if(c->gstream) (void)hipStreamDestroy(c->gstream);
-    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_g[0], c->ev_g[1]})
+    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_g[0], c->ev_g[1], c->ev_g[2], c->ev_g[3]})
         if(e) (void)hipEventDestroy(e);
     delete c;
     return SPHGPU_OK;
@@ -553,23 +571,29 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     if((rc = step_predict(c))) return rc;
     mark(2);
     if((rc = stage_tree(c))) return rc;
-    mark(3);
-    if((rc = stage_pre(c))) return rc;
-    mark(4);
-    // gravity (vector-ALU bound) runs on the second stream while the force
-    // pass (bound by L2 misses on neighbour records) runs here; the gravity
-    // acceleration is added to acc once both are done
+    // Gravity runs on the second stream in two parts: the tree walk (vector-
+    // ALU bound; needs the tree and positions only), started after the
+    // search, alongside the pre-interaction and the force pass (bound by L2
+    // misses on neighbour records), then the recorded leaves' particle pairs
+    // (need h).  The gravity acceleration is added to acc once both streams
+    // are done.
     const bool grav = c.p.gravity_is_valid;
+    mark(3);
+    if((rc = stage_pre(c, grav))) {
+        // later work on the main stream must not overtake the walk
+        if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
+        return rc;
+    }
+    mark(4);
     if(grav) {
         SPHGPU_HIP(hipEventRecord(c.ev_pre, c.stream));
         SPHGPU_HIP(hipStreamWaitEvent(c.gstream, c.ev_pre, 0));
-        if(c.timers_on) (void)hipEventRecord(c.ev_g[0], c.gstream);
-        if((rc = gravity_launch(c, c.gstream))) return rc;
-        if(c.timers_on) (void)hipEventRecord(c.ev_g[1], c.gstream);
+        if(c.timers_on) (void)hipEventRecord(c.ev_g[2], c.gstream);
+        if((rc = gravity_leaves(c, c.gstream))) return rc;
+        if(c.timers_on) (void)hipEventRecord(c.ev_g[3], c.gstream);
         SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
     }
     if((rc = stage_force(c))) {
-        // later work on the main stream must not overtake the walk
         if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
         return rc;
     }
@@ -593,8 +617,14 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
             // with gravity on, its timer is the walk's own span on its
             // stream, which overlaps the force stage; the wait for its end
             // and the combine (ev[5] to ev[6]) are in no stage
-            if(k == 5 && grav) (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
-            else (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            if(k == 5 && grav) {
+                float ms2 = 0.0f;
+                (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
+                (void)hipEventElapsedTime(&ms2, c.ev_g[2], c.ev_g[3]);
+                ms += ms2;
+            } else {
+                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            }
             c.timer_ms[k] += ms;
         }
     }

@@ -76,6 +76,13 @@ struct __align__(64) GravNode {
     int pad;
 };
 static_assert(sizeof(GravNode) == 64, "GravNode is one 64-byte record");
+// An opened leaf recorded by the split gravity walk: its particle range and
+// the wave's lanes that opened it.
+struct __align__(16) LeafHit {
+    int b, e;
+    uint64_t mask;
+};
+constexpr int LEAFHIT_CAP = 256;  // recorded leaves per wave before it falls back
 template <class T> struct GravPart;
 template <> struct __align__(32) GravPart<float> {  // mixed: FP32 mirrors of m and 2/h
     double x[3];
@@ -163,7 +170,7 @@ struct Ctx {
     // second stream (default priority) on which sphgpu_integrate runs the
     // gravity pass alongside the force pass
     hipStream_t gstream = nullptr;
-    hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[2] = {nullptr, nullptr};
+    hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[4] = {nullptr, nullptr, nullptr, nullptr};
     int device = 0;
     int num_cu = 256;         // compute units of the device
     int *work = nullptr;      // per-XCD task counters of persistent kernels
@@ -193,6 +200,8 @@ struct Ctx {
     double4 *msum = nullptr;   // raw mass moments (sum m x, sum m)
     GravNode *gnode = nullptr; // gravity node records (gravity on)
     double *gacc = nullptr;    // gravity's acceleration, D x cap_n (gravity on)
+    LeafHit *leafhit = nullptr; // split walk: LEAFHIT_CAP records per wave
+    int *leafcnt = nullptr;     // split walk: records per wave
     double4 *posrec = nullptr; // force pass: x, y, z, h per particle
     void *pair_rec = nullptr;  // records of the running pair pass: PreRec<T>
                                // (pre-interaction) or ForceRec<T> (force)
@@ -282,11 +291,15 @@ int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV swit
 // force.hip
 int force_compute(Ctx &c);
 // gravity.hip
+// The gravity pass in three parts, each on stream st: the walk (monopoles
+// into phi and gacc, and a record of the opened leaves; needs positions and
+// the tree only), the recorded leaves' particle pairs (need h; added to
+// phi and gacc), and the combine acc += gacc (after the force pass has
+// assigned acc).  gravity_compute runs all three on c.stream;
+// sphgpu_integrate overlaps the first two with the hydrodynamic stages.
 int gravity_compute(Ctx &c);
-// The gravity pass in two parts: the walk on stream st writes phi and its
-// acceleration into gacc; the combine adds gacc to acc (after the force
-// pass has assigned acc).  gravity_compute runs both on c.stream.
-int gravity_launch(Ctx &c, hipStream_t st);
+int gravity_walk(Ctx &c, hipStream_t st);
+int gravity_leaves(Ctx &c, hipStream_t st);
 int gravity_combine(Ctx &c, hipStream_t st);
 // Zero the NUM_XCD task counters at c.work + NUM_XCD * slot before a
 // persistent launch (slot 0: the neighbour search).

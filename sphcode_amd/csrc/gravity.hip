@@ -27,6 +27,8 @@ struct GravArgs {
     const void *gpart;
     double *gacc;        // D x stride, the walk's acceleration
     int64_t stride;
+    LeafHit *leafhit;    // split walk records, LEAFHIT_CAP per wave
+    int *leafcnt;
 };
 
 // acc += gravity's acceleration (after the force pass assigned acc)
@@ -79,7 +81,9 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_parts_ke
     out[i] = p;  // i == n: the spare record
 }
 
-template <int D, class T> struct GravityVisitor {
+// RECORD: the walk of the split pass, which records each opened leaf (with
+// the lanes that opened it) instead of summing its particles.
+template <int D, class T, bool RECORD = false> struct GravityVisitor {
     const BoxDev &box;
     const GravPart<T> *gp;
     double G, theta2;
@@ -87,6 +91,8 @@ template <int D, class T> struct GravityVisitor {
     T einv_i;   // 2 / h_i
     Acc<T> phi;
     Acc<T> acc[D];
+    LeafHit *rec = nullptr;  // RECORD: this wave's records
+    int nrec = 0;            // RECORD: leaves opened so far (may pass the capacity)
 
     __device__ __forceinline__ GravityVisitor(const BoxDev &b_, const GravPart<T> *gp_) : box(b_), gp(gp_) {}
 
@@ -118,6 +124,18 @@ template <int D, class T> struct GravityVisitor {
 
     __device__ __forceinline__ void leaf(int b, int e, bool open)
     {
+        if constexpr(RECORD) {
+            const uint64_t m = uniform64(__ballot(open));
+            if(nrec < LEAFHIT_CAP && wave_lane() == 0) {
+                LeafHit h;
+                h.b = b;
+                h.e = e;
+                h.mask = m;
+                rec[nrec] = h;
+            }
+            ++nrec;
+            return;
+        }
         const T w = open ? T(1) : T(0);
         for(int j = b; j < e; ++j) {
             const GravPart<T> p = gp[j];
@@ -157,22 +175,20 @@ template <int D, class T> struct GravityVisitor {
     }
 };
 
-template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+// Split pass, part 1: the walk with monopoles only (no h needed), which
+// records the opened leaves per wave for part 2.
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_walk_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
+    const int task = uniform(i / WAVE);
     const bool valid = i < n;
-    GravityVisitor<D, T> v(box, (const GravPart<T> *)a.gpart);
+    GravityVisitor<D, T, true> v(box, nullptr);
     v.G = a.G;
     v.theta2 = a.theta2;
     v.einv_i = T(0);
-    if(valid) {
+    v.rec = a.leafhit + (int64_t)task * LEAFHIT_CAP;
 #pragma unroll
-        for(int k = 0; k < D; ++k) v.xi[k] = s.f[F_X + k][i];
-        v.einv_i = (T)(2.0 * s.f[F_HINV][i]);
-    } else {
-#pragma unroll
-        for(int k = 0; k < D; ++k) v.xi[k] = 0.0;
-    }
+    for(int k = 0; k < D; ++k) v.xi[k] = valid ? s.f[F_X + k][i] : 0.0;
     __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
@@ -183,6 +199,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         for(int k = 0; k < D; ++k) a.gacc[k * a.stride + i] = v.acc[k].val();
     }
     if(wave_lane() == 0) {
+        a.leafcnt[task] = v.nrec;
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)wc.pairs);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)wc.accepted);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)wc.opened);
@@ -190,15 +207,86 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     }
 }
 
-template <class T> static void gravity_t(Ctx &c, GravArgs a, const BoxDev &box, hipStream_t st)
+// Split pass, part 2: the particle pairs of the recorded leaves (softened
+// with h_i and h_j), added to part 1's sums.  A wave whose record overflowed
+// walks the tree again in full and overwrites part 1's sums.
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_leaf_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
-    const int n = (int)c.n;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
+    const int task = uniform(i / WAVE);
+    const bool valid = i < n;
+    const int nrec = uniform(a.leafcnt[task]);
+    double xi[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) xi[k] = valid ? s.f[F_X + k][i] : 0.0;
+    const T einv_i = valid ? (T)(2.0 * s.f[F_HINV][i]) : T(0);
+    if(nrec > LEAFHIT_CAP) {
+        GravityVisitor<D, T> v(box, (const GravPart<T> *)a.gpart);
+        v.G = a.G;
+        v.theta2 = a.theta2;
+        v.einv_i = einv_i;
+#pragma unroll
+        for(int k = 0; k < D; ++k) v.xi[k] = xi[k];
+        __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
+        __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
+        const int w = uniform(threadIdx.x / WAVE);
+        (void)tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+        if(valid) {
+            s.f[F_PHI][i] = v.phi.val();
+#pragma unroll
+            for(int k = 0; k < D; ++k) a.gacc[k * a.stride + i] = v.acc[k].val();
+        }
+        return;
+    }
+    GravityVisitor<D, T> v(box, (const GravPart<T> *)a.gpart);
+    v.G = a.G;
+    v.theta2 = a.theta2;
+    v.einv_i = einv_i;
+#pragma unroll
+    for(int k = 0; k < D; ++k) v.xi[k] = xi[k];
+    const LeafHit *rec = a.leafhit + (int64_t)task * LEAFHIT_CAP;
+    const int lane = wave_lane();
+    for(int k = 0; k < nrec; ++k) {
+        const LeafHit h = rec[k];
+        v.leaf(h.b, h.e, (h.mask >> lane) & 1ull);
+    }
+    if(valid) {
+        s.f[F_PHI][i] += v.phi.val();
+#pragma unroll
+        for(int k = 0; k < D; ++k) a.gacc[k * a.stride + i] += v.acc[k].val();
+    }
+}
+
+static GravArgs grav_args(Ctx &c)
+{
+    GravArgs a{};
+    a.G = c.p.gravity_constant;
+    a.theta2 = c.p.gravity_theta * c.p.gravity_theta;
+    a.counters = c.counters;
     a.gnode = c.gnode;
     a.gpart = c.gpart;
     a.gacc = c.gacc;
     a.stride = c.cap_n;
+    a.leafhit = c.leafhit;
+    a.leafcnt = c.leafcnt;
+    return a;
+}
+
+template <class T> static void walk_t(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
+{
+    const int n = (int)c.n;
     hipLaunchKernelGGL(pack_nodes_kernel, dim3(std::min<int64_t>(grid_for(c.node_cap), 2048)), dim3(BLOCK), 0, st, c.t,
                        c.gnode, a.G, c.node_cap);
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL((gravity_walk_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_walk_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_walk_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    }
+}
+
+template <class T> static void leaves_t(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
+{
+    const int n = (int)c.n;
     GravPart<T> *gp = (GravPart<T> *)c.gpart;
     switch(c.D) {
     case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
@@ -206,23 +294,31 @@ template <class T> static void gravity_t(Ctx &c, GravArgs a, const BoxDev &box, 
     default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
     }
     switch(c.D) {
-    case 1: hipLaunchKernelGGL((gravity_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    case 2: hipLaunchKernelGGL((gravity_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    default: hipLaunchKernelGGL((gravity_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 1: hipLaunchKernelGGL((gravity_leaf_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_leaf_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_leaf_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
     }
 }
 
-int gravity_launch(Ctx &c, hipStream_t st)
+int gravity_walk(Ctx &c, hipStream_t st)
 {
     if(!c.p.gravity_is_valid) return SPHGPU_OK;
-    if(!c.gnode || !c.gpart || !c.gacc) return set_error(SPHGPU_E_STATE, "gravity: no node records");
-    GravArgs a{};
-    a.G = c.p.gravity_constant;
-    a.theta2 = c.p.gravity_theta * c.p.gravity_theta;
-    a.counters = c.counters;
+    if(!c.gnode || !c.gpart || !c.gacc || !c.leafhit) return set_error(SPHGPU_E_STATE, "gravity: no node records");
+    const GravArgs a = grav_args(c);
     const BoxDev box = make_box(c);
-    if(c.p.precision == SPHGPU_MIXED) gravity_t<float>(c, a, box, st);
-    else gravity_t<double>(c, a, box, st);
+    if(c.p.precision == SPHGPU_MIXED) walk_t<float>(c, a, box, st);
+    else walk_t<double>(c, a, box, st);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int gravity_leaves(Ctx &c, hipStream_t st)
+{
+    if(!c.p.gravity_is_valid) return SPHGPU_OK;
+    const GravArgs a = grav_args(c);
+    const BoxDev box = make_box(c);
+    if(c.p.precision == SPHGPU_MIXED) leaves_t<float>(c, a, box, st);
+    else leaves_t<double>(c, a, box, st);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
@@ -240,9 +336,12 @@ int gravity_combine(Ctx &c, hipStream_t st)
     return SPHGPU_OK;
 }
 
+// The stage call (GravityForce::calculation): both parts and the combine
+// in order on the context stream, the same arithmetic as sphgpu_integrate.
 int gravity_compute(Ctx &c)
 {
-    int rc = gravity_launch(c, c.stream);
+    int rc = gravity_walk(c, c.stream);
+    if(!rc) rc = gravity_leaves(c, c.stream);
     return rc ? rc : gravity_combine(c, c.stream);
 }
 

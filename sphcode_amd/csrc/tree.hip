@@ -475,6 +475,10 @@ int tree_alloc(Ctx &c)
         // one spare record: a scalar load may fetch a whole record past the last
         SPHGPU_HIP(hipMalloc(&c.gpart, (n + 1) * sizeof(GravPart<double>)));
         SPHGPU_HIP(hipMalloc(&c.gacc, (size_t)c.D * n * sizeof(double)));
+        // one record slot per wave of a grid_for(n)-block launch
+        const size_t waves = (size_t)grid_for(n) * (BLOCK / WAVE);
+        SPHGPU_HIP(hipMalloc(&c.leafhit, waves * LEAFHIT_CAP * sizeof(LeafHit)));
+        SPHGPU_HIP(hipMalloc(&c.leafcnt, waves * sizeof(int)));
     }
     if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
     if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, 2 * NLEVEL * sizeof(int), hipHostMallocDefault));
