@@ -129,6 +129,19 @@ __device__ __forceinline__ int wave_incl_max_nonneg(int x)
 // Lane l receives lane l-1's value; lane 0 receives 0 (DPP wave_shr:1).
 __device__ __forceinline__ int wave_shr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
 
+// Read of a record the kernel never writes, through the constant address
+// space: a wave-uniform address then becomes a scalar load even where the
+// kernel also stores through other pointers (which otherwise forces vector
+// loads).  The host compilation pass only needs a plain read.
+template <class T> __device__ __forceinline__ T load_ro(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p;
+#endif
+}
+
 // |r| in FP32 of a gather candidate from its FP64 r^2 (mixed mode): the
 // hardware FP64 square root (one instruction, no refinement sequence)
 // rounded to FP32, which is the correctly rounded FP32 |r| except when the

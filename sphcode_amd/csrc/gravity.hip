@@ -138,7 +138,7 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
         }
         const T w = open ? T(1) : T(0);
         for(int j = b; j < e; ++j) {
-            const GravPart<T> p = gp[j];
+            const GravPart<T> p = load_ro(gp + j);
             double xj[D];
 #pragma unroll
             for(int k = 0; k < D; ++k) xj[k] = p.x[k];
@@ -247,7 +247,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     const LeafHit *rec = a.leafhit + (int64_t)task * LEAFHIT_CAP;
     const int lane = wave_lane();
     for(int k = 0; k < nrec; ++k) {
-        const LeafHit h = rec[k];
+        const LeafHit h = load_ro(rec + k);
         v.leaf(h.b, h.e, (h.mask >> lane) & 1ull);
     }
     if(valid) {

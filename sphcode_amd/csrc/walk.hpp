@@ -75,7 +75,7 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
         }
         if(node >= total) break;
         ++wc.visits;
-        const GravNode g = gn[node];
+        const GravNode g = load_ro(gn + node);
         const int skip = g.skip;
         const bool me = (active >> lane) & 1ull;
         const bool open = v.node(g, me);
