@@ -159,6 +159,8 @@ typedef struct sphgpu_stats {
     int64_t gravity_wave_steps;   /* iterations of the gravity walk, summed over
                                      waves (each visits one node per iteration
                                      for the lanes whose traversal is there)   */
+    int64_t gravity_rewalks;      /* 64-particle groups whose record of opened
+                                     leaves overflowed, walked again in full   */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

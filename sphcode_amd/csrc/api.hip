@@ -793,6 +793,7 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->search_nodes = k[C_SEARCH_NODES];
     st->search_candidates = k[C_SEARCH_CAND];
     st->gravity_wave_steps = k[C_GRAV_STEPS];
+    st->gravity_rewalks = k[C_GRAV_REWALK];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];

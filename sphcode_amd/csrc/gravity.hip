@@ -200,6 +200,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     }
     if(wave_lane() == 0) {
         a.leafcnt[task] = v.nrec;
+        if(v.nrec > LEAFHIT_CAP) atomicAdd((unsigned long long *)&a.counters[C_GRAV_REWALK], 1ull);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)wc.pairs);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)wc.accepted);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)wc.opened);

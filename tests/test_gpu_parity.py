@@ -155,6 +155,34 @@ def test_three_steps(gpu, case):
     assert _rms_err(b["acc"], a["acc"]) <= 1e-5, f"{case}: acc err {_rms_err(b['acc'], a['acc']):.3e}"
 
 
+def test_gravity_leaf_record_overflow(gpu):
+    """leafParticleNumber 1 (the reference's default, src/solver.cpp:241):
+    every opened leaf is one particle, so the walk's per-wave record of
+    opened leaves (256 entries) overflows and those groups are walked again
+    in full by the leaf pass.  The result must still be the oracle's phi
+    and acceleration, at initialize and after a step (the other gravity
+    cases cover groups whose record fits)."""
+    from sphcode_amd import ics
+    p, f = ics.evrard(4096, leaf=1)
+    ref = OracleEngine(p)
+    from sphcode_amd import GPUEngine
+    gpu_e = GPUEngine(p)
+    for e in (ref, gpu_e):
+        e.upload(f)
+        e.initialize()
+    st = gpu_e.stats()
+    print(f"[overflow] rewalks={st['gravity_rewalks']} groups={len(f['mass']) // 64}", flush=True)
+    assert st["gravity_rewalks"] > 0
+    for step in range(2):
+        a = ref.download()
+        b = gpu_e.download()
+        assert _rel(b["phi"], a["phi"]) <= 1e-9, f"step {step}: phi rel err {_rel(b['phi'], a['phi']):.3e}"
+        assert _rms_err(b["acc"], a["acc"]) <= 1e-8, f"step {step}: acc err {_rms_err(b['acc'], a['acc']):.3e}"
+        ref.integrate()
+        gpu_e.integrate()
+    gpu_e.close()
+
+
 def test_midrun_state_parity(gpu):
     """A state with nonzero velocities and evolved alpha, dt > 0 (TDAV)."""
     from sphcode_amd import GPUEngine
