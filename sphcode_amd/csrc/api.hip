@@ -81,7 +81,7 @@ static void free_buffers(Ctx &c)
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
     fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
-    fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.leafhit); fr(c.leafcnt); fr(c.posrec); fr(c.pair_rec);
+    fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.leafhit); fr(c.leafcnt); fr(c.posrec); fr(c.pair_rec); fr(c.grad_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
@@ -107,6 +107,8 @@ static int alloc_buffers(Ctx &c, int64_t n)
     static_assert(sizeof(ForceRec<float>) >= sizeof(PreRec<float>) && sizeof(ForceRec<double>) >= sizeof(PreRec<double>),
                   "pair_rec holds either record");
     SPHGPU_HIP(hipMalloc(&c.pair_rec, np * (mixed ? sizeof(ForceRec<float>) : sizeof(ForceRec<double>))));
+    if(c.p.sph_type == SPHGPU_GSPH && c.p.gsph_is_2nd_order)
+        SPHGPU_HIP(hipMalloc(&c.grad_rec, np * (mixed ? sizeof(GradRec<float>) : sizeof(GradRec<double>))));
     SPHGPU_HIP(hipMalloc(&c.s.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.alt.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.s.nb, np * sizeof(int)));

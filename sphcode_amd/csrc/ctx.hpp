@@ -102,6 +102,14 @@ template <class T> struct __align__(16) ForceRec {
 };
 static_assert(sizeof(ForceRec<float>) == 48 && sizeof(ForceRec<double>) == 96, "ForceRec size");
 
+// GSPH second order: the MUSCL gradients of a particle (density,
+// pressure, velocity component c along l at gv[3c + l]) in one record.
+template <class T> struct __align__(16) GradRec {
+    T gd[3], gp[3], gv[9];
+    T pad;
+};
+static_assert(sizeof(GradRec<float>) == 64 && sizeof(GradRec<double>) == 128, "GradRec size");
+
 // Neighbour records of the pre-interaction's final sweep: FP64 position,
 // and m, u, c, v in the pair type.
 template <class T> struct PreRec;
@@ -205,6 +213,7 @@ struct Ctx {
     double4 *posrec = nullptr; // force pass: x, y, z, h per particle
     void *pair_rec = nullptr;  // records of the running pair pass: PreRec<T>
                                // (pre-interaction) or ForceRec<T> (force)
+    void *grad_rec = nullptr;  // GSPH second order: GradRec<T> per particle
     void *gpart = nullptr;     // GravPart<T> per particle (gravity on)
     // search groups: particle ranges of at most 64 particles, each inside
     // the deepest cell holding at most 64 particles

@@ -28,12 +28,14 @@ struct ForceArgs {
     int *rec_cnt;
     const double4 *posrec;  // x, y, z, h (FP64)
     const void *frec;       // ForceRec<T>
+    const void *grec;       // GradRec<T> (GSPH second order)
 };
 
 // One pass over the particles: the records the pair loop reads for j, with
 // the fields the formulation reads (FP32 mirrors in mixed mode).
 template <int D, int TYPE, class T>
-__global__ void __launch_bounds__(BLOCK) pack_force_kernel(State s, int n, int use_u, double4 *pr, ForceRec<T> *fr)
+__global__ void __launch_bounds__(BLOCK) pack_force_kernel(State s, int n, int use_u, double4 *pr, ForceRec<T> *fr,
+                                                           GradRec<T> *gr)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
@@ -58,6 +60,19 @@ __global__ void __launch_bounds__(BLOCK) pack_force_kernel(State s, int n, int u
     }
     if(use_u) r.u = fld<T>(s, F_ENE, i);
     fr[i] = r;
+    if constexpr(TYPE == SPHGPU_GSPH) {
+        if(gr) {
+            GradRec<T> g{};
+#pragma unroll
+            for(int l = 0; l < D; ++l) {
+                g.gd[l] = fld<T>(s, F_GD + l, i);
+                g.gp[l] = fld<T>(s, F_GP + l, i);
+#pragma unroll
+                for(int c = 0; c < D; ++c) g.gv[3 * c + l] = fld<T>(s, F_GV + 3 * c + l, i);
+            }
+            gr[i] = g;
+        }
+    }
 }
 
 // van Leer limiter (src/gsph/g_fluid_force.cpp:27-36).
@@ -186,14 +201,16 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     const T di = T(0.5) * (T(1) - ci * dt * rinv);
                     const T dj = T(0.5) * (T(1) - cj * dt * rinv);
                     const T dv = ve_i - ve_j;
+                    const GradRec<T> *grec = (const GradRec<T> *)a.grec;
+                    const GradRec<T> gri = grec[i], grj = grec[j];
                     T gvi[D], gvj[D];
 #pragma unroll
                     for(int c = 0; c < D; ++c) {
-                        T si = fld<T>(s, F_GV + 3 * c, i) * e[0], sj = fld<T>(s, F_GV + 3 * c, j) * e[0];
+                        T si = gri.gv[3 * c] * e[0], sj = grj.gv[3 * c] * e[0];
 #pragma unroll
                         for(int l = 1; l < D; ++l) {
-                            si += fld<T>(s, F_GV + 3 * c + l, i) * e[l];
-                            sj += fld<T>(s, F_GV + 3 * c + l, j) * e[l];
+                            si += gri.gv[3 * c + l] * e[l];
+                            sj += grj.gv[3 * c + l] * e[l];
                         }
                         gvi[c] = si;
                         gvj[c] = sj;
@@ -202,14 +219,14 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     const T dve_j = dotd<D, T>(gvj, e) * r;
                     R[0] = ve_i - van_leer(dv, dve_i) * di;
                     L[0] = ve_j + van_leer(dv, dve_j) * dj;
-                    T gdi = fld<T>(s, F_GD, i) * e[0], gdj = fld<T>(s, F_GD, j) * e[0];
-                    T gpi = fld<T>(s, F_GP, i) * e[0], gpj = fld<T>(s, F_GP, j) * e[0];
+                    T gdi = gri.gd[0] * e[0], gdj = grj.gd[0] * e[0];
+                    T gpi = gri.gp[0] * e[0], gpj = grj.gp[0] * e[0];
 #pragma unroll
                     for(int l = 1; l < D; ++l) {
-                        gdi += fld<T>(s, F_GD + l, i) * e[l];
-                        gdj += fld<T>(s, F_GD + l, j) * e[l];
-                        gpi += fld<T>(s, F_GP + l, i) * e[l];
-                        gpj += fld<T>(s, F_GP + l, j) * e[l];
+                        gdi += gri.gd[l] * e[l];
+                        gdj += grj.gd[l] * e[l];
+                        gpi += gri.gp[l] * e[l];
+                        gpj += grj.gp[l] * e[l];
                     }
                     const T dd = rhoi - rhoj;
                     R[1] = rhoi - van_leer(dd, gdi * r) * di;
@@ -293,7 +310,7 @@ template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, cons
 {
     const int use_u = TYPE == SPHGPU_DISPH || a.use_ac;
     hipLaunchKernelGGL((pack_force_kernel<D, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, use_u,
-                       c.posrec, (ForceRec<T> *)c.pair_rec);
+                       c.posrec, (ForceRec<T> *)c.pair_rec, (GradRec<T> *)c.grad_rec);
     hipLaunchKernelGGL((force_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
@@ -334,6 +351,7 @@ int force_compute(Ctx &c)
     a.rec_cnt = c.record ? c.rec_force_cnt : nullptr;
     a.posrec = c.posrec;
     a.frec = c.pair_rec;
+    a.grec = c.grad_rec;
     const BoxDev box = make_box(c);
     if(c.p.precision == SPHGPU_MIXED) force_t<float>(c, a, box);
     else force_t<double>(c, a, box);
