@@ -341,19 +341,25 @@ template <class T> __device__ __forceinline__ T soft_pot(T r, T einv)
 }
 
 // Both softened kernels at once, without branches, given u = r/e, 1/e and
-// 1/r (1/r may be 0 at r = 0, where only the inner piece is used).
+// 1/r (1/r may be 0 at r = 0, where only the inner piece is used); the
+// polynomials of src/bhtree.cpp:273-299 in Horner form:
+//   inner  f = (1.4 + u^2 (-1/6 + u^2 (3/40 - u/40))) / e
+//          g = (4/3 + u^2 (-6/5 + u/2)) / e^3
+//   middle f = -1/(15 r) + (1.6 + u^2 (-4/3 + u (1 + u (-3/10 + u/30)))) / e
+//          g = (-1/15 + u^3 (8/3 + u (-3 + u (6/5 - u/6)))) / r^3
 template <class T> __device__ __forceinline__ void soft_both(T u, T einv, T rinv, T &pot, T &acc)
 {
     const T u2 = u * u;
     const T u3 = u2 * u;
     const T e3 = einv * einv * einv;
     const T r3 = rinv * rinv * rinv;
-    const T pot_in = (T(-0.5) * u2 * (T(1.0 / 3.0) - T(3.0 / 20.0) * u2 + u3 * T(1.0 / 20.0)) + T(1.4)) * einv;
-    const T acc_in = (T(4.0 / 3.0) - T(1.2) * u2 + T(0.5) * u3) * e3;
-    const T pot_mid = T(-1.0 / 15.0) * rinv + (-u2 * (T(4.0 / 3.0) - u + T(0.3) * u2 - u3 * T(1.0 / 30.0)) + T(1.6)) * einv;
-    const T acc_mid = (T(-1.0 / 15.0) + T(8.0 / 3.0) * u3 - T(3) * u3 * u + T(1.2) * u3 * u2 - u3 * u3 * T(1.0 / 6.0)) * r3;
-    pot = u < T(1) ? pot_in : (u < T(2) ? pot_mid : rinv);
-    acc = u < T(1) ? acc_in : (u < T(2) ? acc_mid : r3);
+    const T pot_in = (T(1.4) + u2 * (T(-1.0 / 6.0) + u2 * (T(3.0 / 40.0) - u * T(1.0 / 40.0)))) * einv;
+    const T acc_in = (T(4.0 / 3.0) + u2 * (T(-1.2) + u * T(0.5))) * e3;
+    const T pot_mid = T(-1.0 / 15.0) * rinv + (T(1.6) + u2 * (T(-4.0 / 3.0) + u * (T(1) + u * (T(-0.3) + u * T(1.0 / 30.0))))) * einv;
+    const T acc_mid = (T(-1.0 / 15.0) + u3 * (T(8.0 / 3.0) + u * (T(-3) + u * (T(1.2) - u * T(1.0 / 6.0))))) * r3;
+    const bool in = u < T(1), mid = u < T(2);
+    pot = in ? pot_in : (mid ? pot_mid : rinv);
+    acc = in ? acc_in : (mid ? acc_mid : r3);
 }
 
 template <class T> __device__ __forceinline__ T soft_acc(T r, T einv)
