@@ -331,15 +331,6 @@ template <int D, class T> struct KernelOf<D, 1, T> { using type = WendlandC4<D, 
 // ------------------------------------------- Hernquist & Katz softening
 // f = potential kernel and g = force kernel of src/bhtree.cpp:273-299 with
 // e = h/2, u = r/e; g(r) * r_ij is the softened r_ij / r^3.
-template <class T> __device__ __forceinline__ T soft_pot(T r, T einv)
-{
-    const T u = r * einv;
-    const T u2 = u * u;
-    if(u < T(1)) return (T(-0.5) * u2 * (T(1.0 / 3.0) - T(3.0 / 20.0) * u2 + u2 * u * T(1.0 / 20.0)) + T(1.4)) * einv;
-    if(u < T(2)) return T(-1.0 / 15.0) / r + (-u2 * (T(4.0 / 3.0) - u + T(0.3) * u2 - u2 * u * T(1.0 / 30.0)) + T(1.6)) * einv;
-    return T(1) / r;
-}
-
 // Both softened kernels at once, without branches, given u = r/e, 1/e and
 // 1/r (1/r may be 0 at r = 0, where only the inner piece is used); the
 // polynomials of src/bhtree.cpp:273-299 in Horner form:
@@ -362,17 +353,5 @@ template <class T> __device__ __forceinline__ void soft_both(T u, T einv, T rinv
     acc = in ? acc_in : (mid ? acc_mid : r3);
 }
 
-template <class T> __device__ __forceinline__ T soft_acc(T r, T einv)
-{
-    const T u = r * einv;
-    const T u2 = u * u;
-    if(u < T(1)) return (T(4.0 / 3.0) - T(1.2) * u2 + T(0.5) * u2 * u) * (einv * einv * einv);
-    const T r3inv = T(1) / (r * r * r);
-    if(u < T(2)) {
-        const T u3 = u2 * u;
-        return (T(-1.0 / 15.0) + T(8.0 / 3.0) * u3 - T(3) * u3 * u + T(1.2) * u3 * u2 - u3 * u3 * T(1.0 / 6.0)) * r3inv;
-    }
-    return r3inv;
-}
 
 }  // namespace sphgpu
