@@ -136,7 +136,15 @@ __device__ __forceinline__ int wave_shr1(int x) { return __builtin_amdgcn_update
 template <class T> __device__ __forceinline__ T load_ro(const T *p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(4))) T *)p;
+    // word by word: a whole-struct copy is split by field type, and some
+    // pieces were still selected as vector loads
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "load_ro reads 16-byte words");
+    T out;
+    const __attribute__((address_space(4))) uint4 *q = (const __attribute__((address_space(4))) uint4 *)p;
+    uint4 *o = reinterpret_cast<uint4 *>(&out);
+#pragma unroll
+    for(int k = 0; k < (int)(sizeof(T) / 16); ++k) o[k] = q[k];
+    return out;
 #else
     return *p;
 #endif
