@@ -67,15 +67,13 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
     // parked at total have rejoined first).  `active` only shrinks to the
     // lanes opening a node, which are never none, so a wave with a valid
     // lane keeps one.
-    if(active == 0) return wc;
-    while(true) {
-        while(node == top_node) {
-            active |= top_mask;
-            pop();
-        }
-        if(node >= total) break;
+    // The loop tests for the end at the bottom only, so the visitor's sums
+    // stay in the same registers from one visit to the next.
+    if(active == 0 || total <= 0) return wc;
+    const char *gnb = reinterpret_cast<const char *>(gn);
+    do {
         ++wc.visits;
-        const GravNode g = load_ro(gn + node);
+        const GravNode g = load_ro(reinterpret_cast<const GravNode *>(gnb + ((uint32_t)node << 6)));
         const int skip = g.skip;
         const bool me = (active >> lane) & 1ull;
         const bool open = v.node(g, me);
@@ -110,7 +108,11 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
         } else {
             node = skip;
         }
-    }
+        while(node == top_node) {
+            active |= top_mask;
+            pop();
+        }
+    } while(node < total);
     return wc;
 }
 
