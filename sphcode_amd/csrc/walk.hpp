@@ -71,11 +71,11 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
     // stay in the same registers from one visit to the next.
     if(active == 0 || total <= 0) return wc;
     const char *gnb = reinterpret_cast<const char *>(gn);
+    bool me = valid;  // this lane's bit of `active`, kept alongside it
     do {
         ++wc.visits;
         const GravNode g = load_ro(reinterpret_cast<const GravNode *>(gnb + ((uint32_t)node << 6)));
         const int skip = g.skip;
-        const bool me = (active >> lane) & 1ull;
         const bool open = v.node(g, me);
         const uint64_t om = uniform64(__ballot(open));
         const int nopen = __popcll(om);
@@ -104,12 +104,14 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
                 }
             }
             active = om;
+            me = open;
             node = node + 1;
         } else {
             node = skip;
         }
         while(node == top_node) {
             active |= top_mask;
+            me = me || ((top_mask >> lane) & 1ull);
             pop();
         }
     } while(node < total);
