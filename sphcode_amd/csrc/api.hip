@@ -151,6 +151,7 @@ static int stage_tree(Ctx &c)
     int rc = tree_build(c);
     if(rc) return rc;
     c.tree_ready = true;
+    c.gpart_fresh = false;
     c.have_gather = c.have_force = false;
     c.lists_symmetric = false;  // lists refer to the previous order
     return SPHGPU_OK;

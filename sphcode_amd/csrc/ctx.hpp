@@ -247,6 +247,7 @@ struct Ctx {
     // host mirror of simulation scalars
     double time = 0.0, dt = 0.0, h_per_v_sig = 0.0;
     bool uploaded = false, first_pre = true, tree_ready = false;
+    bool gpart_fresh = false;  // the last pre-interaction wrote gpart for this tree
 
     // differentiable rollout: per-step checkpoints (device order, with the
     // id permutation of that moment) and adjoint buffers

@@ -288,11 +288,15 @@ template <class T> static void walk_t(Ctx &c, const GravArgs &a, const BoxDev &b
 template <class T> static void leaves_t(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
 {
     const int n = (int)c.n;
+    // the pre-interaction writes the particle records with h; pack them
+    // here only if it has not run on this tree
     GravPart<T> *gp = (GravPart<T> *)c.gpart;
-    switch(c.D) {
-    case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
-    case 2: hipLaunchKernelGGL((pack_parts_kernel<2, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
-    default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+    if(!c.gpart_fresh) {
+        switch(c.D) {
+        case 1: hipLaunchKernelGGL((pack_parts_kernel<1, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+        case 2: hipLaunchKernelGGL((pack_parts_kernel<2, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+        default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
+        }
     }
     switch(c.D) {
     case 1: hipLaunchKernelGGL((gravity_leaf_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;

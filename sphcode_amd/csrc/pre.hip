@@ -44,6 +44,7 @@ struct PreArgs {
     int64_t *counters;
     double *partial;      // per-block min of h / v_sig
     const void *prec;     // PreRec<T> per particle
+    void *gpart;          // GravPart<T> per particle, written with h (gravity on)
 };
 
 // Records read for j by the final sweep (FP32 mirrors in mixed mode).
@@ -259,6 +260,15 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const double rho_d = rho.val();
         put<T>(s, F_SML, i, h);
         put<T>(s, F_HINV, i, 1.0 / h);
+        if(a.gpart) {
+            // the gravity leaf pass's record of i, as pack_parts_kernel writes it
+            GravPart<T> gp{};
+#pragma unroll
+            for(int k = 0; k < 3; ++k) gp.x[k] = k < D ? xi[k] : 0.0;
+            gp.m = (T)mi;
+            gp.einv = T(2) * (T)(1.0 / h);  // 2 x the FP32 mirror of 1/h in mixed mode
+            ((GravPart<T> *)a.gpart)[i] = gp;
+        }
         put<T>(s, F_DENS, i, rho_d);
         double pres;
         if constexpr(TYPE == SPHGPU_DISPH) {
@@ -424,11 +434,13 @@ int pre_compute(Ctx &c)
 {
     int rc = mirror_fields(c, kPreInputs, kNumPreInputs);
     if(rc) return rc;
-    const PreArgs a = make_args(c);
+    PreArgs a = make_args(c);
+    a.gpart = c.p.gravity_is_valid ? c.gpart : nullptr;
     const BoxDev box = make_box(c);
     if(c.p.precision == SPHGPU_MIXED) pre_t<float>(c, a, box);
     else pre_t<double>(c, a, box);
     SPHGPU_HIP(hipGetLastError());
+    c.gpart_fresh = a.gpart != nullptr;
     return reduce_min_partials(c, grid_for(c.n), S_HVS);
 }
 
