@@ -58,25 +58,35 @@ __device__ __forceinline__ int xcd_block() { return uniform(xcd_block((int)block
 
 // Persistent waves: tasks [0, ntask) are split into NUM_XCD contiguous
 // shares, and the waves of each XCD take the tasks of its share one at a
-// time from that XCD's counter (hardware deals block b to XCD b mod 8).  A
-// wave's next task is -1 when its share is used up.  Call with all lanes
-// active.
+// time from that XCD's counter (hardware deals block b to XCD b mod 8).
+// A wave whose share is used up moves on to the next XCD's share, so no
+// XCD idles while another still has tasks; after visiting all NUM_XCD
+// shares it gets -1.  Call with all lanes active.
 struct TaskShare {
-    int *ctr;
+    int *work;
+    int ntask;
+    int x, visited = 0;
     int lo, hi;
-    __device__ __forceinline__ TaskShare(int *work, int ntask)
+    __device__ __forceinline__ void set(int xcd)
     {
-        const int x = (int)(blockIdx.x % NUM_XCD);
-        ctr = work + x;
+        x = xcd;
         lo = (int)((int64_t)ntask * x / NUM_XCD);
         hi = (int)((int64_t)ntask * (x + 1) / NUM_XCD);
     }
+    __device__ __forceinline__ TaskShare(int *work_, int ntask_) : work(work_), ntask(ntask_)
+    {
+        set((int)(blockIdx.x % NUM_XCD));
+    }
     __device__ __forceinline__ int next()
     {
-        int t = 0;
-        if(wave_lane() == 0) t = atomicAdd(ctr, 1);
-        t = lo + __builtin_amdgcn_readfirstlane(t);
-        return t < hi ? t : -1;
+        while(true) {
+            int t = 0;
+            if(wave_lane() == 0) t = atomicAdd(work + x, 1);
+            t = lo + __builtin_amdgcn_readfirstlane(t);
+            if(t < hi) return t;
+            if(++visited == NUM_XCD) return -1;
+            set(x + 1 < NUM_XCD ? x + 1 : 0);
+        }
     }
 };
 
