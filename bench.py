@@ -254,7 +254,9 @@ def main() -> int:
                 "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s,
                 # node visits of the lanes over 64 x the waves' walk iterations
                 "walk_lane_utilisation": (stats["gravity_opened"] + stats["gravity_pn"]) /
-                                         max(64 * stats["gravity_wave_steps"], 1)}
+                                         max(64 * stats["gravity_wave_steps"], 1),
+                # particle pairs over the lane slots the pair pass evaluates
+                "pair_lane_utilisation": stats["gravity_pp"] / max(stats["gravity_pp_slots"], 1)}
 
     adjoint = None
     if not args.no_adjoint:

@@ -161,6 +161,10 @@ typedef struct sphgpu_stats {
                                      for the lanes whose traversal is there)   */
     int64_t gravity_rewalks;      /* 64-particle groups whose record of opened
                                      leaves overflowed, walked again in full   */
+    int64_t gravity_pp_slots;     /* lane slots of the particle-pair pass: 64
+                                     per particle of each leaf that a 64-lane
+                                     group opened (gravity_pp / this is the
+                                     pass's lane efficiency)                    */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

@@ -98,7 +98,8 @@ class Stats(C.Structure):
         "steps", "gather_pairs", "force_pairs", "gravity_pp", "gravity_pn",
         "newton_failures", "newton_iterations", "search_truncations",
         "list_overflows", "max_list_length", "tree_nodes", "tree_max_level", "gravity_opened",
-        "search_nodes", "search_candidates", "gravity_wave_steps", "gravity_rewalks")]
+        "search_nodes", "search_candidates", "gravity_wave_steps", "gravity_rewalks",
+        "gravity_pp_slots")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -797,6 +797,7 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->search_candidates = k[C_SEARCH_CAND];
     st->gravity_wave_steps = k[C_GRAV_STEPS];
     st->gravity_rewalks = k[C_GRAV_REWALK];
+    st->gravity_pp_slots = k[C_GRAV_SLOTS];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];

@@ -90,6 +90,13 @@ struct TaskShare {
     }
 };
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <class T> __device__ __forceinline__ T wave_sum(T v)
 {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

@@ -163,7 +163,7 @@ __device__ __forceinline__ int list_slot(int kk, int front, int cap) { return kk
 // Device-side counters (int64), reduced per wave before the atomic.
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
-    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, C_GRAV_REWALK, NCOUNTERS
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, C_GRAV_REWALK, C_GRAV_SLOTS, NCOUNTERS
 };
 
 // Device scalars.

@@ -137,41 +137,45 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
             return;
         }
         const T w = open ? T(1) : T(0);
-        for(int j = b; j < e; ++j) {
-            const GravPart<T> p = load_ro(gp + j);
-            double xj[D];
+        for(int j = b; j < e; ++j) pair(load_ro(gp + j), w);
+    }
+
+    // The pair (i, j) with weight w: 1, or 0 for a lane that only keeps the
+    // wave in step (its sums are then unchanged).
+    __device__ __forceinline__ void pair(const GravPart<T> &p, T w)
+    {
+        double xj[D];
 #pragma unroll
-            for(int k = 0; k < D; ++k) xj[k] = p.x[k];
-            const T mj = p.m;
-            const T einv_j = p.einv;
-            double dd[D];
-            const double r2d = sep2<D>(box, xi, xj, dd);
-            T rij[D];
-            T r2 = T(0);
+        for(int k = 0; k < D; ++k) xj[k] = p.x[k];
+        const T mj = p.m;
+        const T einv_j = p.einv;
+        double dd[D];
+        const double r2d = sep2<D>(box, xi, xj, dd);
+        T rij[D];
+        T r2 = T(0);
 #pragma unroll
-            for(int k = 0; k < D; ++k) {
-                rij[k] = (T)dd[k];
-                r2 += rij[k] * rij[k];
-            }
-            if constexpr(sizeof(T) == 8) r2 = r2d;
-            const T rinv = r2 > T(0) ? rsqrt_t<T>(r2) : T(0);
-            const T rr = r2 * rinv;
-            // far field of both softening lengths unless one is within 2e
-            T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
-            const T ui = rr * einv_i, uj = rr * einv_j;
-            if(ui < T(2) || uj < T(2)) {
-                T pi_, ai, pj, aj;
-                soft_both(ui, einv_i, rinv, pi_, ai);
-                soft_both(uj, einv_j, rinv, pj, aj);
-                pot = pi_ + pj;
-                acc3 = ai + aj;
-            }
-            const T gm = (T)G * mj * T(0.5) * w;
-            phi.add(-(gm * pot));
-            const T c = gm * acc3;
-#pragma unroll
-            for(int k = 0; k < D; ++k) acc[k].add_prod(dd[k], -c);
+        for(int k = 0; k < D; ++k) {
+            rij[k] = (T)dd[k];
+            r2 += rij[k] * rij[k];
         }
+        if constexpr(sizeof(T) == 8) r2 = r2d;
+        const T rinv = r2 > T(0) ? rsqrt_t<T>(r2) : T(0);
+        const T rr = r2 * rinv;
+        // far field of both softening lengths unless one is within 2e
+        T pot = rinv + rinv, acc3 = T(2) * rinv * rinv * rinv;
+        const T ui = rr * einv_i, uj = rr * einv_j;
+        if(ui < T(2) || uj < T(2)) {
+            T pi_, ai, pj, aj;
+            soft_both(ui, einv_i, rinv, pi_, ai);
+            soft_both(uj, einv_j, rinv, pj, aj);
+            pot = pi_ + pj;
+            acc3 = ai + aj;
+        }
+        const T gm = (T)G * mj * T(0.5) * w;
+        phi.add(-(gm * pot));
+        const T c = gm * acc3;
+#pragma unroll
+        for(int k = 0; k < D; ++k) acc[k].add_prod(dd[k], -c);
     }
 };
 
@@ -205,6 +209,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)wc.accepted);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)wc.opened);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)(unsigned)wc.visits);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)wc.slots);
     }
 }
 
@@ -245,11 +250,62 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     v.einv_i = einv_i;
 #pragma unroll
     for(int k = 0; k < D; ++k) v.xi[k] = xi[k];
+    // Each lane runs through its own leaves only: the records' lane masks
+    // are transposed into per-lane record masks, 64 records per word, and
+    // every lane steps through the (leaf, particle) sequence of its set
+    // bits at its own pace, so no lane waits on leaves that it did not
+    // open.  A lane adds its pairs in the same order as a sweep over all
+    // records would (records in walk order, particles in tree order).
     const LeafHit *rec = a.leafhit + (int64_t)task * LEAFHIT_CAP;
     const int lane = wave_lane();
-    for(int k = 0; k < nrec; ++k) {
-        const LeafHit h = load_ro(rec + k);
-        v.leaf(h.b, h.e, (h.mask >> lane) & 1ull);
+    const int w = uniform(threadIdx.x / WAVE);
+    __shared__ int2 srec[BLOCK / WAVE][LEAFHIT_CAP];
+    constexpr int NW = LEAFHIT_CAP / WAVE;
+    static_assert(NW == 4, "four record words per lane");
+    const int nwords = (nrec + WAVE - 1) / WAVE;
+    uint64_t mine[NW];
+#pragma unroll
+    for(int c = 0; c < NW; ++c) {
+        mine[c] = 0;
+        if(c < nwords) {
+            const int k = c * WAVE + lane;
+            uint64_t m = 0;
+            if(k < nrec) {
+                const LeafHit h = rec[k];
+                m = h.mask;
+                srec[w][k] = make_int2(h.b, h.e);
+            }
+            for(int l = 0; l < WAVE; ++l) {
+                const uint64_t col = uniform64(__ballot((m >> l) & 1ull));
+                if(lane == l) mine[c] = col;
+            }
+        }
+    }
+    wave_lds_sync();
+    const GravPart<T> *gp = (const GravPart<T> *)a.gpart;
+    int c = 0, j = 0, e = 0;
+    uint64_t cur = mine[0];
+    bool live = true;
+    while(true) {
+        // next leaf of this lane once its current one is done
+        while(live && j == e) {
+            while(cur == 0 && c + 1 < nwords) {
+                ++c;
+                cur = c == 1 ? mine[1] : (c == 2 ? mine[2] : mine[NW - 1]);
+            }
+            if(cur == 0) {
+                live = false;
+            } else {
+                const int r = (int)__builtin_ctzll(cur);
+                cur &= cur - 1;
+                const int2 be = srec[w][c * WAVE + r];
+                j = be.x;
+                e = be.y;
+            }
+        }
+        if(uniform64(__ballot(live)) == 0) break;
+        v.pair(gp[live ? j : 0], live ? T(1) : T(0));
+        ++j;
     }
     if(valid) {
         s.f[F_PHI][i] += v.phi.val();

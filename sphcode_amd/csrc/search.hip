@@ -66,13 +66,6 @@ __device__ __forceinline__ int wave_exclusive_scan(int v, int &total)
     return x - v;
 }
 
-// Orders this wave's LDS writes before its later LDS reads by other lanes.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // Minimum-image separation of one axis (include/periodic.hpp:34-59).
 __device__ __forceinline__ double min_image1(const BoxDev &b, int k, double d)
 {

@@ -34,9 +34,11 @@ constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 // The walk state (cursor, masks, parked stack) is wave-uniform and kept in
 // scalar registers; each node is one 64-byte record (GravNode) read with
 // one scalar load.  WalkCount returns, per wave, the node visits and the
-// lane totals of opened nodes, accepted nodes and particle interactions.
+// lane totals of opened nodes, accepted nodes and particle interactions, and
+// the particles of the leaves that some lane opened.
 struct WalkCount {
     int visits = 0, opened = 0, accepted = 0, pairs = 0;
+    int slots = 0;  // particles of the leaves that some lane opened
 };
 
 template <class V>
@@ -85,6 +87,7 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
             if(om) {
                 v.leaf(g.begin, g.end, open);
                 wc.pairs += nopen * (g.end - g.begin);
+                wc.slots += g.end - g.begin;
             }
             node = skip;
         } else if(om) {
