@@ -162,9 +162,12 @@ typedef struct sphgpu_stats {
     int64_t gravity_rewalks;      /* 64-particle groups whose record of opened
                                      leaves overflowed, walked again in full   */
     int64_t gravity_pp_slots;     /* lane slots of the particle-pair pass: 64
-                                     per particle of each leaf that a 64-lane
-                                     group opened (gravity_pp / this is the
-                                     pass's lane efficiency)                    */
+                                     per iteration of a 64-lane group's pair
+                                     loop (as many as its lane with the most
+                                     pairs has; for a group walked again in
+                                     full, per particle of each leaf that one
+                                     of its lanes opened); gravity_pp / this
+                                     is the pass's lane efficiency             */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

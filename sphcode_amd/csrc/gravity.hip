@@ -197,6 +197,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
     const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+    const int pairs = wave_sum(wc.pairs), accepted = wave_sum(wc.accepted), opened = wave_sum(wc.opened);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
@@ -205,11 +206,10 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     if(wave_lane() == 0) {
         a.leafcnt[task] = v.nrec;
         if(v.nrec > LEAFHIT_CAP) atomicAdd((unsigned long long *)&a.counters[C_GRAV_REWALK], 1ull);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)wc.pairs);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)wc.accepted);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)wc.opened);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)pairs);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)accepted);
+        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)opened);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)(unsigned)wc.visits);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)wc.slots);
     }
 }
 
@@ -236,7 +236,9 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
         __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
         const int w = uniform(threadIdx.x / WAVE);
-        (void)tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+        const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+        if(wave_lane() == 0)
+            atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)wc.slots);
         if(valid) {
             s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
@@ -286,6 +288,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     int c = 0, j = 0, e = 0;
     uint64_t cur = mine[0];
     bool live = true;
+    int iters = 0;
     while(true) {
         // next leaf of this lane once its current one is done
         while(live && j == e) {
@@ -306,7 +309,9 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         if(uniform64(__ballot(live)) == 0) break;
         v.pair(gp[live ? j : 0], live ? T(1) : T(0));
         ++j;
+        ++iters;
     }
+    if(lane == 0) atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)iters);
     if(valid) {
         s.f[F_PHI][i] += v.phi.val();
 #pragma unroll
