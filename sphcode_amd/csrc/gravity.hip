@@ -307,11 +307,17 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
             }
         }
         if(uniform64(__ballot(live)) == 0) break;
-        v.pair(gp[live ? j : 0], live ? T(1) : T(0));
-        ++j;
+        // two consecutive particles of the leaf per pass (one if it has one
+        // left), so that the leaf bookkeeping above runs half as often
+        const bool two = live && j + 1 < e;
+        const GravPart<T> p0 = gp[live ? j : 0];
+        const GravPart<T> p1 = gp[two ? j + 1 : 0];
+        v.pair(p0, live ? T(1) : T(0));
+        v.pair(p1, two ? T(1) : T(0));
+        j += two ? 2 : (live ? 1 : 0);
         ++iters;
     }
-    if(lane == 0) atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)iters);
+    if(lane == 0) atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)(2 * WAVE) * (unsigned)iters);
     if(valid) {
         s.f[F_PHI][i] += v.phi.val();
 #pragma unroll
