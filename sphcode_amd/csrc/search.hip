@@ -115,6 +115,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         }
     }
     const double h2_own = h * h;
+    const int64_t h2_bits = __double_as_longlong(h2_own);
     const size_t lbase = list_base(valid ? i : 0, a.cap);
 
     // group box: centre, half extent, largest radius (wave-uniform)
@@ -270,17 +271,18 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
 #pragma unroll
                     for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
                     const double r2 = sep2<D>(box, xi, xj, d);
-                    double lim = h2_own;
+                    // r2 < max(h_i^2, kl^2) as two tests; all three are
+                    // non-negative, so their bit patterns order as integers
                     if constexpr(IJ) {
-                        const double kl2 = c_lim[w][c];
-                        lim = kl2 > lim ? kl2 : lim;
+                        const int64_t rb = __double_as_longlong(r2);
+                        return (rb < h2_bits) | (rb < __double_as_longlong(c_lim[w][c]));
                     }
-                    return r2 < lim;
+                    else return r2 < h2_own;
                 };
                 uint32_t mlo = 0, mhi = 0;
                 const int m1 = m < 32 ? m : 32;
-                for(int c = 0; c < m1; ++c) mlo |= hit(c) ? 1u << c : 0u;
-                for(int c = 32; c < m; ++c) mhi |= hit(c) ? 1u << (c - 32) : 0u;
+                for(int c = 0; c < m1; ++c) mlo |= (uint32_t)hit(c) << c;
+                for(int c = 32; c < m; ++c) mhi |= (uint32_t)hit(c) << (c - 32);
                 uint64_t mine = ((uint64_t)mhi << 32) | mlo;
                 while(mine) {
                     const int c = __builtin_ctzll(mine);
