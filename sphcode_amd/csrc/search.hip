@@ -51,6 +51,14 @@ struct SearchArgs {
 
 constexpr int SW = BLOCK / WAVE;  // waves per block
 constexpr int LEAFCAP = 128;      // leaves collected per flush
+constexpr int FLATCAP = 256;      // listed (lane, candidate) hits per tile
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, WAVE);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, WAVE);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 // Wave-wide exclusive prefix sum.
 __device__ __forceinline__ int wave_exclusive_scan(int v, int &total)
@@ -88,6 +96,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     __shared__ double c_x[SW][3][WAVE];
     __shared__ double c_lim[SW][WAVE];  // IJ: the candidate's radius squared
     __shared__ int c_j[SW][WAVE];
+    __shared__ uint16_t flat[SW][FLATCAP];
 
     const int w = uniform(threadIdx.x / WAVE);
     const int lane = wave_lane();
@@ -263,30 +272,108 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             const int m = uniform(__popcll(kept));
             tested += m;
             wave_lds_sync();
+            // pass A: every lane tests every staged candidate (uniform
+            // loop, broadcast LDS reads) into a hit mask, and a mask of the
+            // hits inside its own radius (the gather entries of a split
+            // search; every hit otherwise)
+            uint64_t mine = 0, mfront = 0;
             if(valid) {
-                // test every staged candidate (uniform loop, broadcast LDS
-                // reads), then append this lane's hits in staging order
-                auto hit = [&](int c) {
+                uint32_t mlo = 0, mhi = 0, flo = 0, fhi = 0;
+                auto test = [&](int c, uint32_t &hb, uint32_t &fb) {
                     double xj[D], d[D];
 #pragma unroll
                     for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
                     const double r2 = sep2<D>(box, xi, xj, d);
                     // r2 < max(h_i^2, kl^2) as two tests; all three are
                     // non-negative, so their bit patterns order as integers
-                    if constexpr(IJ) {
-                        const int64_t rb = __double_as_longlong(r2);
-                        return (rb < h2_bits) | (rb < __double_as_longlong(c_lim[w][c]));
-                    }
-                    else return r2 < h2_own;
+                    const int64_t rb = __double_as_longlong(r2);
+                    const bool f = rb < h2_bits;
+                    bool h = f;
+                    if constexpr(IJ) h = f | (rb < __double_as_longlong(c_lim[w][c]));
+                    hb = (uint32_t)h;
+                    fb = (uint32_t)f;
                 };
-                uint32_t mlo = 0, mhi = 0;
                 const int m1 = m < 32 ? m : 32;
-                for(int c = 0; c < m1; ++c) mlo |= (uint32_t)hit(c) << c;
-                for(int c = 32; c < m; ++c) mhi |= (uint32_t)hit(c) << (c - 32);
-                uint64_t mine = ((uint64_t)mhi << 32) | mlo;
-                while(mine) {
-                    const int c = __builtin_ctzll(mine);
-                    mine &= mine - 1ull;
+                for(int c = 0; c < m1; ++c) {
+                    uint32_t hb, fb;
+                    test(c, hb, fb);
+                    mlo |= hb << c;
+                    flo |= fb << c;
+                }
+                for(int c = 32; c < m; ++c) {
+                    uint32_t hb, fb;
+                    test(c, hb, fb);
+                    mhi |= hb << (c - 32);
+                    fhi |= fb << (c - 32);
+                }
+                mine = ((uint64_t)mhi << 32) | mlo;
+                mfront = a.split ? ((uint64_t)fhi << 32) | flo : mine;
+            }
+            // pass B: append each lane's hits in staging order, gather
+            // entries from the front of its list and force-only entries
+            // from the back.  The tile's (lane, candidate) hits are listed
+            // in LDS and written 64 at a time, one per lane, each at the
+            // slot its rank among its lane's hits gives; a tile in which
+            // some list would reach its capacity (or with more than FLATCAP
+            // hits) takes the one-lane-at-a-time loop, which keeps the
+            // first cap entries.
+            const int nh = __popcll(mine);
+            const int nf = __popcll(mfront);
+            int htotal = 0;
+            const int hbase = wave_exclusive_scan(nh, htotal);
+            htotal = uniform(htotal);
+            const bool fits = wf + wb + nh <= a.cap;
+            if(uniform64(__ballot(!fits)) == 0 && htotal <= FLATCAP) {
+                {
+                    uint64_t mm = mine;
+                    int k = hbase;
+                    while(mm) {
+                        const int c = __builtin_ctzll(mm);
+                        mm &= mm - 1ull;
+                        flat[w][k++] = (uint16_t)((lane << 8) | c);
+                    }
+                }
+                wave_lds_sync();
+                for(int r = 0; r < htotal; r += WAVE) {
+                    const int p = r + lane;
+                    const bool act = p < htotal;
+                    const int code = act ? (int)flat[w][p] : 0;
+                    const int owner = code >> 8, c = code & 255;
+                    const uint64_t om = shfl64(mine, owner), of = shfl64(mfront, owner);
+                    const int owf = __shfl(wf, owner, WAVE), owb = __shfl(wb, owner, WAVE);
+                    double xo[D];
+#pragma unroll
+                    for(int k = 0; k < D; ++k) xo[k] = __longlong_as_double((long long)shfl64((uint64_t)__double_as_longlong(xi[k]), owner));
+                    if(act) {
+                        const uint64_t below = (1ull << c) - 1ull;
+                        const int jj = c_j[w][c];
+                        const int io = gr.x + owner;
+                        const size_t ob = list_base(io, a.cap);
+                        if((of >> c) & 1ull) {
+                            const size_t at = ob + list_off(owf + __popcll(of & below));
+                            out[at] = jj;
+                            if(outr || outrf) {
+                                double xj[D], d[D];
+#pragma unroll
+                                for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
+                                const double r2 = sep2<D>(box, xo, xj, d);
+                                if(outr) outr[at] = sqrt(r2);
+                                if(outrf) outrf[at] = gather_r_f32(r2);
+                            }
+                        } else {
+                            out[ob + list_off(a.cap - 1 - (owb + __popcll(om & ~of & below)))] = jj;
+                        }
+                    }
+                }
+                wf += nf;
+                wb += nh - nf;
+                count += nf;
+                count2 += nh - nf;
+            } else if(valid) {
+                uint64_t mm = mine;
+                while(mm) {
+                    const int c = __builtin_ctzll(mm);
+                    mm &= mm - 1ull;
                     double xj[D], d[D];
 #pragma unroll
                     for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
