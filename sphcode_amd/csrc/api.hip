@@ -170,6 +170,20 @@ static int launch_gravity_walk(Ctx &c)
     return rc;
 }
 
+// sphgpu_integrate's gravity pair pass on the second stream, after the
+// main stream's work so far (the pre-interaction's h and the force pass's
+// packing) and the walk.
+static int launch_gravity_leaves(Ctx &c)
+{
+    SPHGPU_HIP(hipEventRecord(c.ev_pre, c.stream));
+    SPHGPU_HIP(hipStreamWaitEvent(c.gstream, c.ev_pre, 0));
+    if(c.timers_on) (void)hipEventRecord(c.ev_g[2], c.gstream);
+    int rc = gravity_leaves(c, c.gstream);
+    if(c.timers_on) (void)hipEventRecord(c.ev_g[3], c.gstream);
+    SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
+    return rc;
+}
+
 // The pre-interaction's candidate search.  After the first call, one walk
 // finds both the gather candidates and the symmetric pairs the force stage
 // needs (search_combined); the first call keeps the reference's initial
@@ -588,15 +602,11 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
         return rc;
     }
     mark(4);
-    if(grav) {
-        SPHGPU_HIP(hipEventRecord(c.ev_pre, c.stream));
-        SPHGPU_HIP(hipStreamWaitEvent(c.gstream, c.ev_pre, 0));
-        if(c.timers_on) (void)hipEventRecord(c.ev_g[2], c.gstream);
-        if((rc = gravity_leaves(c, c.gstream))) return rc;
-        if(c.timers_on) (void)hipEventRecord(c.ev_g[3], c.gstream);
-        SPHGPU_HIP(hipEventRecord(c.ev_grav, c.gstream));
-    }
-    if((rc = stage_force(c))) {
+    // the leaf pairs start once the force pass has packed its records
+    c.after_pack = grav ? launch_gravity_leaves : nullptr;
+    rc = stage_force(c);
+    c.after_pack = nullptr;
+    if(rc) {
         if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
         return rc;
     }

@@ -179,6 +179,11 @@ struct Ctx {
     // gravity pass alongside the force pass
     hipStream_t gstream = nullptr;
     hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[4] = {nullptr, nullptr, nullptr, nullptr};
+    // called by the force pass between its record packing and its pair
+    // kernel (sphgpu_integrate starts the gravity pair pass there, so that
+    // the packing is not left waiting behind it); its status is kept here
+    int (*after_pack)(Ctx &) = nullptr;
+    int after_pack_rc = 0;
     int device = 0;
     int num_cu = 256;         // compute units of the device
     int *work = nullptr;      // per-XCD task counters of persistent kernels

@@ -311,6 +311,7 @@ template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, cons
     const int use_u = TYPE == SPHGPU_DISPH || a.use_ac;
     hipLaunchKernelGGL((pack_force_kernel<D, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, use_u,
                        c.posrec, (ForceRec<T> *)c.pair_rec, (GradRec<T> *)c.grad_rec);
+    if(c.after_pack) c.after_pack_rc = c.after_pack(c);
     hipLaunchKernelGGL((force_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
@@ -353,10 +354,11 @@ int force_compute(Ctx &c)
     a.frec = c.pair_rec;
     a.grec = c.grad_rec;
     const BoxDev box = make_box(c);
+    c.after_pack_rc = 0;
     if(c.p.precision == SPHGPU_MIXED) force_t<float>(c, a, box);
     else force_t<double>(c, a, box);
     SPHGPU_HIP(hipGetLastError());
-    return SPHGPU_OK;
+    return c.after_pack_rc;
 }
 
 }  // namespace sphgpu
