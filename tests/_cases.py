@@ -70,6 +70,15 @@ def rollout_ssph():
     return ics.rollout(n_side=12)
 
 
+def rollout_dense():
+    # 200 neighbours in 3D: a search tile then holds far more (lane,
+    # candidate) hits than the batched append takes at once, so the
+    # one-lane-at-a-time append runs beside the batched one
+    p, f = ics.rollout(n_side=10)
+    p.neighbor_number = 200
+    return p, f
+
+
 def sedov_gsph():
     # A milder contrast than the 256^3 benchmark: with u_bg = 1e-5 and only
     # 1728 particles the reference scheme itself drives u negative within
@@ -88,5 +97,6 @@ CASES = {
     "evrard_disph": evrard_disph,
     "evrard_ssph": evrard_ssph,
     "rollout_ssph": rollout_ssph,
+    "rollout_dense": rollout_dense,
     "sedov_gsph": sedov_gsph,
 }
