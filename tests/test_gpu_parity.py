@@ -183,6 +183,33 @@ def test_gravity_leaf_record_overflow(gpu):
     gpu_e.close()
 
 
+@pytest.mark.parametrize("case", ["evrard_disph", "evrard_ssph"])
+def test_gravity_interaction_counts(gpu, case):
+    """The wave-coherent walk keeps each particle's own opening decisions, so
+    it opens, accepts and pairs what the reference's recursive walk does
+    (src/bhtree.cpp:301-331): the counts of opened cells, accepted cells and
+    particle pairs match the oracle's (up to the rare cell whose opening test
+    is decided by the last bit of a centre of mass summed in another order).
+    The pair pass's lane slots cover every pair, two per lane and pass."""
+    p, f = CASES[case]()
+    ref = OracleEngine(p)
+    from sphcode_amd import GPUEngine
+    gpu_e = GPUEngine(p)
+    for e in (ref, gpu_e):
+        e.upload(f)
+        e.initialize()
+    a, b = ref.stats(), gpu_e.stats()
+    print(f"[counts] {case}: oracle pp={a['gravity_pp']} pn={a['gravity_pn']} opened={a['gravity_opened']}; "
+          f"gpu pp={b['gravity_pp']} pn={b['gravity_pn']} opened={b['gravity_opened']} slots={b['gravity_pp_slots']}",
+          flush=True)
+    for k in ("gravity_pp", "gravity_pn", "gravity_opened"):
+        assert a[k] > 0
+        assert abs(b[k] - a[k]) <= 1e-4 * a[k], f"{case}: {k} {b[k]} against {a[k]}"
+    assert b["gravity_rewalks"] == 0
+    assert b["gravity_pp_slots"] >= b["gravity_pp"] and b["gravity_pp_slots"] % 128 == 0
+    gpu_e.close()
+
+
 def test_midrun_state_parity(gpu):
     """A state with nonzero velocities and evolved alpha, dt > 0 (TDAV)."""
     from sphcode_amd import GPUEngine
