@@ -200,8 +200,8 @@ def main() -> int:
         del src, dst
         log(f"[bench] copy calibration: {args.copy_mib} MiB copied")
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design in DESIGN.md, "
-                                   "not an implementation", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding over several GPUs (Morton-key ranges "
+                                   "with halo exchange) is designed but not implemented", "n_gpus": args.gpus}))
         return 2
 
     from sphcode_amd import FP64, MIXED, costs, ics
