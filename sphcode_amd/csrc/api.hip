@@ -297,6 +297,9 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     if(p.dim < 1 || p.dim > 3) return set_error(SPHGPU_E_INVALID, "dim must be 1, 2 or 3");
     if(p.sph_type < 0 || p.sph_type > 2) return set_error(SPHGPU_E_INVALID, "Unknown SPH type");
     if(p.kernel != SPHGPU_CUBIC_SPLINE && p.kernel != SPHGPU_WENDLAND) return set_error(SPHGPU_E_INVALID, "kernel is unknown.");
+    // C4Kernel asserts DIM != 1 (include/kernel/wendland_kernel.hpp:25-28):
+    // its 1D normalisation is 0, so every density would be 0.
+    if(p.kernel == SPHGPU_WENDLAND && p.dim == 1) return set_error(SPHGPU_E_INVALID, "Wendland C4 kernel is not defined for DIM = 1");
     if(p.precision != SPHGPU_FP64 && p.precision != SPHGPU_MIXED) return set_error(SPHGPU_E_INVALID, "unknown precision");
     if(p.tree_max_level < 1 || p.tree_max_level * p.dim > 63)
         return set_error(SPHGPU_E_INVALID, "maxTreeLevel must satisfy 1 <= maxTreeLevel and maxTreeLevel * DIM <= 63");

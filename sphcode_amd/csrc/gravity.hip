@@ -98,8 +98,11 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
 
     // Opening test edge^2 > theta^2 |x_i - com|^2 (FP64, all lanes); a lane
     // of `me` that does not open adds the monopole, the others add zero.
+    // |x_i - com|^2 is summed left to right without contraction, as the
+    // reference's inner product is, so both take the same cells.
     __device__ __forceinline__ bool node(const GravNode &g, bool me)
     {
+#pragma clang fp contract(off)
         double d[D];
         d[0] = xi[0] - g.cx;
         if constexpr(D > 1) d[1] = xi[1] - g.cy;
@@ -107,7 +110,7 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
         min_image<D>(box, d);
         double d2 = d[0] * d[0];
 #pragma unroll
-        for(int k = 1; k < D; ++k) d2 += d[k] * d[k];
+        for(int k = 1; k < D; ++k) d2 = d2 + d[k] * d[k];
         const bool o = g.edge2 > theta2 * d2;
         const bool far = me && !o;
         T gmn;

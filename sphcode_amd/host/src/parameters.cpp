@@ -115,6 +115,9 @@ std::shared_ptr<SPHParameters> read_parameters(const std::string &text, std::str
     if (kernel == "cubic_spline") p->kernel = KernelType::CUBIC_SPLINE;
     else if (kernel == "wendland") p->kernel = KernelType::WENDLAND;
     else throw SPHException("kernel is unknown.");
+    // C4Kernel asserts DIM != 1 (include/kernel/wendland_kernel.hpp:25-28)
+    if (p->kernel == KernelType::WENDLAND && p->dim == 1)
+        throw SPHException("Wendland C4 kernel is not defined for DIM = 1");
 
     p->iterative_sml = get_bool(in, "iterativeSmoothingLength", true);
 

@@ -90,6 +90,19 @@ def test_create_rejects_bad_parameters(field, value, code):
     assert fns["last_error"]()
 
 
+def test_wendland_in_one_dimension_rejected():
+    """The reference's C4Kernel asserts DIM != 1
+    (include/kernel/wendland_kernel.hpp:25-28): its 1D normalisation is 0."""
+    lib = C.CDLL(str(LIB))
+    fns = _abi.bind(lib, "sphgpu_", _abi.GPU_ONLY)
+    p = _abi.default_params(1)
+    p.gamma = 1.4
+    p.kernel = _abi.WENDLAND
+    ctx = C.c_void_p()
+    assert fns["create"](C.byref(p), C.byref(ctx)) == _abi.E_INVALID
+    assert b"Wendland" in fns["last_error"]()
+
+
 def test_time_dependent_av_alpha_order_rejected():
     """alphaMax < alphaMin is an error (src/solver.cpp:227-228)."""
     lib = C.CDLL(str(LIB))

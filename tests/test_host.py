@@ -73,6 +73,7 @@ def test_required_keys(tmp_path, key):
     ({"useGravity": "yes"}, "conversion of data to type failed (useGravity)"),
     ({"DIM": 4}, "DIM must be 1, 2 or 3"),
     ({"precision": "fp16"}, "Unknown precision"),
+    ({"DIM": 1, "kernel": "wendland"}, "Wendland C4 kernel is not defined for DIM = 1"),
 ])
 def test_errors(tmp_path, doc, msg):
     rc, _, err = parse(tmp_path, {**REQUIRED, **doc})
