@@ -34,29 +34,57 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(params, fields, n: int, steps: int) -> dict:
+def host_cpu() -> dict:
+    """CPU model, sockets, cores per socket and threads per core (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "logical_cpus": info.get("CPU(s)")}
+
+
+def cpu_baseline(params, fields, n: int, steps: int, warmup: int = 1) -> dict:
     """Time the oracle (reference flags: -O3 -funroll-loops -ffast-math
-    -fopenmp) on this host: initialize untimed, then `steps` timed steps."""
+    -fopenmp) on this host's cores: initialize and `warmup` steps untimed,
+    then `steps` timed steps, with the oracle's per-stage seconds."""
+    import numpy as np
     from sphcode_amd import _abi
     from sphcode_amd.engine import Engine
     lib = C.CDLL(str(ROOT / "oracle" / "lib" / "libsphref3_fast.so"))
-    fns = _abi.bind(lib, "sphref_", {"set_threads": (C.c_int, [C.c_void_p, C.c_int])})
+    fns = _abi.bind(lib, "sphref_", {"set_threads": (C.c_int, [C.c_void_p, C.c_int]),
+                                      "stage_seconds": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)])})
     e = Engine(fns, params, name="sphref")
     threads = fns["set_threads"](None, 0)
     e.upload(fields)
     t0 = time.perf_counter()
     e.initialize()
     log(f"[cpu] initialize {time.perf_counter() - t0:.2f} s with {threads} threads")
+    for _ in range(warmup):
+        e.integrate()
+    e.reset_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
         e.integrate()
     dt = time.perf_counter() - t0
+    st = np.zeros(7)
+    e._call("stage_seconds", st.ctypes.data_as(C.POINTER(C.c_double)))
     e.close()
     rate = n * steps / dt
-    log(f"[cpu] {steps} step(s) {dt:.2f} s -> {rate:.4g} particle-steps/s")
+    stages = dict(zip(("timestep", "predict", "tree", "pre_interaction", "fluid_force", "gravity", "correct"),
+                      (float(x) / steps for x in st)))
+    log(f"[cpu] {steps} step(s) {dt:.2f} s -> {rate:.4g} particle-steps/s; per-stage s/step {stages}")
     return {"value": rate, "unit": "particle-steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle built with the reference's flags, Evrard N={n}, FP64: {steps} full step(s) after "
-                      f"initialize on {threads} OpenMP threads ({dt:.1f} s timed)"}
+            "sample": f"oracle built with the reference's flags, Evrard N={n}, FP64: initialize and {warmup} "
+                      f"warm-up step(s) untimed, then {steps} timed steps on {threads} OpenMP threads "
+                      f"({dt:.1f} s timed)",
+            "host": host_cpu(), "stage_s_per_step": stages}
 
 
 def timed_run(params, fields, steps: int, warmup: int, label: str):
@@ -145,6 +173,70 @@ def workload_rate(name: str, params, fields, steps: int, warmup: int) -> dict:
             "force_pairs_per_s": stats["force_pairs"] / el}
 
 
+def first_step_rate(name: str, params, fields, repeats: int) -> dict:
+    """Time of the first integrate() after Solver::initialize: one untimed
+    warm-up repetition, then `repeats` timed ones, each from a fresh upload
+    of the same initial condition; the median is reported."""
+    import statistics
+
+    from sphcode_amd import GPUEngine
+    n = int(len(fields["mass"]))
+    g = GPUEngine(params)
+    times, stats, timers = [], None, None
+    for r in range(repeats + 1):
+        g.upload(fields)
+        g.initialize()
+        g.synchronize()
+        g.reset_stats()
+        g.enable_timers(True)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.integrate()
+        g.synchronize()
+        el = time.perf_counter() - t0
+        if r > 0:
+            times.append(el)
+            stats, timers = g.stats(), g.timers()
+    g.close()
+    el = statistics.median(times)
+    log(f"[bench] {name}: first step {[round(t * 1e3, 2) for t in times]} ms -> {n / el:.4g} particle-steps/s")
+    return {"n_particles": n, "steps": 1, "repeats": repeats, "particle_steps_per_s": n / el,
+            "ms_per_step": el * 1e3, "stage_ms_per_step": timers, "force_pairs_per_s": stats["force_pairs"] / el}
+
+
+def kernel_rooflines(n: int, stats: dict, timers: dict, steps: int, peak_valu: float) -> dict:
+    """Per-kernel figures from the kernel-span timers: the neighbour search
+    and the two hydrodynamic pair passes against the HBM peak (algorithmic
+    bytes: positions and radii in, the neighbour set out; the stage bytes of
+    the pre-interaction and force), and the pair passes and gravity kernels
+    against the vector peak (algorithmic FLOPs from the pair counters)."""
+    from sphcode_amd import costs
+    pre_f, force_f = costs.pair_flops(stats, n, steps, "disph")
+    out = {}
+
+    def entry(name, ms, nbytes=None, flops=None):
+        ms = ms / steps
+        e = {"launch_ms": ms}
+        if nbytes is not None:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            e.update({"bound": "hbm", "bytes_per_launch": nbytes, "achieved_GBs": gbs,
+                      "frac_hbm": gbs / costs.PEAK_HBM_GBS})
+        if flops is not None:
+            tf = flops / (ms * 1e-3) / 1e12
+            e.update({"flops_per_launch": flops, "achieved_TFLOPs": tf, "frac_valu": tf / peak_valu})
+        out[name] = e
+
+    entry("search_kernel", timers["search"], nbytes=costs.search_bytes(n, stats, steps))
+    entry("pre_kernel", timers["pre_pairs"], nbytes=costs.BYTES_PER_PARTICLE["pre_interaction"] * n, flops=pre_f)
+    entry("force_kernel", timers["force_pairs"], nbytes=costs.BYTES_PER_PARTICLE["fluid_force"] * n, flops=force_f)
+    walk_f = (costs.GRAV_FLOP_PN * stats["gravity_pn"] +
+              costs.GRAV_FLOP_OPEN * (stats["gravity_pn"] + stats["gravity_opened"])) / steps
+    entry("gravity_walk_kernel", timers["gravity_walk"], flops=walk_f)
+    entry("gravity_leaf_kernel", timers["gravity_leaf"], flops=costs.GRAV_FLOP_PP * stats["gravity_pp"] / steps)
+    log("[bench] kernels: " + ", ".join(f"{k} {v['launch_ms']:.3f} ms" for k, v in out.items()))
+    return out
+
+
 def mixed_errors(params_fp64, params_mixed, fields) -> dict:
     """Per-field error of mixed against FP64 after Solver::initialize on the
     benchmark's initial state."""
@@ -179,7 +271,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 run and the error check")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout timing")
@@ -240,16 +332,12 @@ def main() -> int:
     flops = costs.gravity_flops(stats) / args.steps
     achieved = flops / grav_s / 1e12 if grav_s > 0 else 0.0
     peak = costs.PEAK_FP32_TFLOPS if args.precision == "mixed" else costs.PEAK_FP64_TFLOPS
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_gravity_evrard_{args.precision}.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
     roofline = {"bound": "valu", "kernel": "gravity_walk_kernel<3> + gravity_leaf_kernel<3>", "achieved": achieved,
                 "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                # HBM bytes come from rocprofv3 counter runs, which this run
+                # does not make (the profile summaries hold them)
+                "traffic": None,
                 "flops_per_launch": flops, "launch_ms": grav_s * 1e3,
                 "interactions_per_s": (stats["gravity_pp"] + stats["gravity_pn"]) / args.steps / grav_s,
                 # node visits of the lanes over 64 x the waves' walk iterations
@@ -257,6 +345,7 @@ def main() -> int:
                                          max(64 * stats["gravity_wave_steps"], 1),
                 # particle pairs over the lane slots the pair pass evaluates
                 "pair_lane_utilisation": stats["gravity_pp"] / max(stats["gravity_pp_slots"], 1)}
+    kernels = kernel_rooflines(n, stats, timers, args.steps, peak)
 
     adjoint = None
     if not args.no_adjoint:
@@ -266,17 +355,7 @@ def main() -> int:
         adjoint["fp64"] = adj64
         adjoint["grad_rel_err_vs_fp64"] = {
             name: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for name, a, b in zip(("pos", "vel", "ene"), gmx, g64)}
-        # The difference sits on a few particles at the non-smooth points of
-        # the viscosity and Balsara switch, where the two trajectories fall on
-        # different sides; the rest of the field agrees to round-off.
-        rest = {}
-        for name, a, b in zip(("pos", "vel", "ene"), gmx, g64):
-            e = np.sqrt(((a - b).reshape(len(b), -1) ** 2).sum(axis=1))
-            keep = np.argsort(e)[:-100]
-            rest[name] = float(np.sqrt((e[keep] ** 2).sum()) / np.linalg.norm(b))
-        adjoint["grad_rel_err_vs_fp64_without_100_largest"] = rest
-        log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}, without the 100 "
-            f"largest differences: {rest}")
+        log(f"[bench] adjoint mixed vs FP64 gradient: {adjoint['grad_rel_err_vs_fp64']}")
 
     workloads = None
     if not args.no_workloads:
@@ -291,12 +370,14 @@ def main() -> int:
         except Exception as e:  # a side measurement must not lose the main line
             workloads["khi_disph_2d"] = {"error": str(e)}
         try:
-            # Only the first step is timed: with u_bg = 1e-5 the reference
-            # scheme leaves non-finite states after it (the MUSCL-reconstructed
-            # pressure of the blast front goes negative), as the oracle does.
+            # Only the first step from the initial condition is timed: with
+            # u_bg = 1e-5 the reference scheme itself leaves non-finite states
+            # after it (predict drives u of cold particles beside the blast
+            # below zero, src/solver.cpp:449-452).  It is timed from fresh
+            # uploads after one untimed warm-up repetition.
             ps, fs = ics.sedov(n_side=256, precision=MIXED)
             ps.neighbor_capacity = 256
-            workloads["sedov_gsph_3d"] = workload_rate("sedov", ps, fs, 1, 0)
+            workloads["sedov_gsph_3d"] = first_step_rate("sedov", ps, fs, repeats=3)
             del fs
             ps2, fs2 = ics.sedov(n_side=128, precision=FP64)
             ps2.neighbor_capacity = 256
@@ -334,6 +415,7 @@ def main() -> int:
                         "force": stats["force_pairs"] / args.steps / (ms * 1e-3)},
         "stage_ms_per_step": {k: v / args.steps for k, v in timers.items()},
         "roofline": roofline,
+        "kernel_rooflines": kernels,
         "adjoint": adjoint,
         "other_workloads": workloads,
         "cpu_baseline": cpu,

@@ -171,9 +171,17 @@ typedef struct sphgpu_stats {
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),
-   accumulated since the last sphgpu_reset_stats(). */
+   accumulated over sphgpu_integrate calls since the last
+   sphgpu_reset_stats().  The last five are spans of single kernels (with
+   their small record-packing launches) inside those stages:
+     search        the neighbour search of the pre-interaction stage
+     pre_pairs     the pre-interaction's pair pass (Newton h, sums)
+     force_pairs   the fluid force's pair pass
+     gravity_walk  the gravity tree walk (on the gravity stream)
+     gravity_leaf  the gravity particle-pair pass (on the gravity stream) */
 typedef struct sphgpu_timers {
     double timestep, predict, tree, pre_interaction, fluid_force, gravity, correct;
+    double search, pre_pairs, force_pairs, gravity_walk, gravity_leaf;
 } sphgpu_timers;
 
 /* ---------------------------------------------------------------------- */

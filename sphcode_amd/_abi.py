@@ -107,7 +107,8 @@ class Stats(C.Structure):
 
 class Timers(C.Structure):
     _fields_ = [(name, C.c_double) for name in (
-        "timestep", "predict", "tree", "pre_interaction", "fluid_force", "gravity", "correct")]
+        "timestep", "predict", "tree", "pre_interaction", "fluid_force", "gravity", "correct",
+        "search", "pre_pairs", "force_pairs", "gravity_walk", "gravity_leaf")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -26,7 +26,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPHGPU_ARCH", "gfx950")
 
-HOST_SOURCES = ["json.cpp", "parameters.cpp", "simulation.cpp", "modules.cpp", "solver.cpp", "snapshot.cpp"]
+HOST_SOURCES = ["json.cpp", "parameters.cpp", "simulation.cpp", "modules.cpp", "solver.cpp", "snapshot.cpp", "output.cpp"]
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-I", str(ROOT / "include"), "-I", str(PKG / "host" / "include")]
 

@@ -85,6 +85,11 @@ static void free_buffers(Ctx &c)
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
+    fr(c.tile_cand); fr(c.tile_mf); fr(c.tile_mb); fr(c.tile_nc); fr(c.ws_j); fr(c.ws_lim);
+    c.tile_wcap = 0;
+    c.tile_groups_cap = 0;
+    c.ws_waves = 0;
+    c.ngroups = 0;
     fr(c.rec_gather); fr(c.rec_gather_cnt); fr(c.rec_force); fr(c.rec_force_cnt);
     fr(c.partial);
     c.cap_n = 0;
@@ -113,13 +118,14 @@ static int alloc_buffers(Ctx &c, int64_t n)
     SPHGPU_HIP(hipMalloc(&c.alt.id, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.s.nb, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.alt.nb, np * sizeof(int)));
+    // per-particle lists are expanded from the candidate tiles only for the
+    // adjoint (allocated then, ensure_lists)
     const size_t lsz = (size_t)np * (size_t)c.cap;
-    SPHGPU_HIP(hipMalloc(&c.list, lsz * sizeof(int)));
-    if(mixed) SPHGPU_HIP(hipMalloc(&c.list_rf, lsz * sizeof(float)));
-    else SPHGPU_HIP(hipMalloc(&c.list_r, lsz * sizeof(double)));
     SPHGPU_HIP(hipMalloc(&c.list_cnt, np * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.list_cnt2, np * sizeof(int)));
-    const int64_t npart = std::max<int64_t>(grid_for(np), 1024) * 8 + 64;
+    // block partials: per 256 particles, or per block of four search groups
+    // (up to one group per particle)
+    const int64_t npart = std::max<int64_t>(grid_for(np), 1024) * 8 + np / 4 + 64;
     SPHGPU_HIP(hipMalloc(&c.partial, npart * sizeof(double)));
     int rc = tree_alloc(c);
     if(rc) return rc;
@@ -134,6 +140,14 @@ static int alloc_buffers(Ctx &c, int64_t n)
 
 // Reads the error counters; a list overflow or node-pool exhaustion since
 // the last check becomes an error status.
+// Neighbour lists in list_at layout (the adjoint's input), allocated on
+// first use.
+static int ensure_lists(Ctx &c)
+{
+    if(!c.list) SPHGPU_HIP(hipMalloc(&c.list, (size_t)c.cap_n * (size_t)c.cap * sizeof(int)));
+    return SPHGPU_OK;
+}
+
 static int check_flags(Ctx &c, const char *stage)
 {
     SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
@@ -143,6 +157,9 @@ static int check_flags(Ctx &c, const char *stage)
     if(c.counters_host[C_OVERFLOW] > 0)
         return set_error(SPHGPU_E_LIST_OVERFLOW, std::string(stage) + ": neighbour list capacity (" + std::to_string(c.cap) +
                                                      ") exceeded; raise neighbor_capacity");
+    if(c.counters_host[C_TILE_OVER] > 0)
+        return set_error(SPHGPU_E_LIST_OVERFLOW, std::string(stage) + ": candidates of a search group exceed " +
+                                                     std::to_string(c.tile_wcap) + "; raise neighbor_capacity");
     return SPHGPU_OK;
 }
 
@@ -191,6 +208,12 @@ static int launch_gravity_leaves(Ctx &c)
 // walk is started on the second stream once the search is queued, so that
 // it runs alongside the pre-interaction rather than the search (both the
 // walk and the search are bound by vector-ALU issue).
+// Kernel-span marks of sphgpu_integrate's timers (ev[8 + k]).
+static void kmark(Ctx &c, int k)
+{
+    if(c.kernel_marks) (void)hipEventRecord(c.ev[8 + k], c.stream);
+}
+
 static int stage_pre(Ctx &c, bool walk_after_search = false)
 {
     int rc;
@@ -200,20 +223,26 @@ static int stage_pre(Ctx &c, bool walk_after_search = false)
         if((rc = search_gather(c, true))) return rc;
         if((rc = pre_initial(c))) return rc;
         c.first_pre = false;
+        kmark(c, 0);
         if((rc = search_gather(c, false))) return rc;
+        kmark(c, 1);
     } else {
         if((rc = search_guess(c))) return rc;
         if((rc = tree_set_kernel(c))) return rc;  // largest search radius per cell
+        kmark(c, 0);
         if((rc = search_combined(c))) return rc;
+        kmark(c, 1);
     }
     if(c.record) {
-        const size_t lsz = (size_t)c.cap_n * (size_t)c.cap;
-        SPHGPU_HIP(hipMemcpyAsync(c.rec_gather, c.list, lsz * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
-        SPHGPU_HIP(hipMemcpyAsync(c.rec_gather_cnt, c.list_cnt, c.cap_n * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+        if((rc = search_expand(c, c.rec_gather, c.rec_gather_cnt, nullptr, true))) return rc;
         c.have_gather = true;
     }
+    if(c.need_lists && ((rc = ensure_lists(c)) || (rc = search_expand(c, c.list, c.list_cnt, c.list_cnt2, false))))
+        return rc;
     if(walk_after_search && (rc = launch_gravity_walk(c))) return rc;
+    kmark(c, 2);
     if((rc = pre_compute(c))) return rc;
+    kmark(c, 3);
     c.lists_symmetric = false;
     if(combined) {
         // the combined lists hold every force pair unless some h outgrew its
@@ -231,9 +260,15 @@ static int stage_pre(Ctx &c, bool walk_after_search = false)
 static int stage_force(Ctx &c)
 {
     int rc;
-    if(!c.lists_symmetric && (rc = search_symmetric(c))) return rc;
+    if(!c.lists_symmetric) {
+        if((rc = search_symmetric(c))) return rc;
+        if(c.need_lists && ((rc = ensure_lists(c)) || (rc = search_expand(c, c.list, c.list_cnt, c.list_cnt2, false))))
+            return rc;
+    }
     c.lists_symmetric = false;
+    kmark(c, 4);
     if((rc = force_compute(c))) return rc;
+    kmark(c, 5);
     if(c.record) c.have_force = true;
     return SPHGPU_OK;
 }
@@ -584,6 +619,7 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     auto mark = [&](int k) {
         if(c.timers_on) (void)hipEventRecord(ev[k], c.stream);
     };
+    c.kernel_marks = c.timers_on;
     mark(0);
     if(fixed_dt > 0.0) hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, c.stream, c.scal, (int)S_DT, fixed_dt);
     else if((rc = step_timestep(c))) return rc;
@@ -621,6 +657,7 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     mark(6);
     if((rc = step_correct(c))) return rc;
     mark(7);
+    c.kernel_marks = false;
     SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     if((rc = check_flags(c, "integrate"))) return rc;  // synchronises the stream
     c.dt = c.scal_host[S_DT];
@@ -642,6 +679,19 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
                 (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
             }
             c.timer_ms[k] += ms;
+        }
+        // kernel spans: search, pre-interaction pairs, force pairs
+        for(int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[8 + 2 * k], ev[9 + 2 * k]);
+            c.timer_ms[7 + k] += ms;
+        }
+        if(grav) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
+            c.timer_ms[10] += ms;
+            (void)hipEventElapsedTime(&ms, c.ev_g[2], c.ev_g[3]);
+            c.timer_ms[11] += ms;
         }
     }
     return SPHGPU_OK;
@@ -733,6 +783,14 @@ int sphgpu_rollout_adjoint(void *ptr, const double *dl_dx, const double *dl_dv, 
     if(c.ro.steps < 1 || c.ro.n != c.n) return set_error(SPHGPU_E_STATE, "rollout adjoint: no stored rollout");
     const int K = c.ro.steps;
     if((rc = adjoint_seed(c, dl_dx, dl_dv, dl_du))) return rc;
+    if((rc = ensure_lists(c))) return rc;
+    // the recomputed steps expand their candidate tiles into lists, which
+    // the adjoint passes read
+    struct ListsOn {
+        Ctx &c;
+        explicit ListsOn(Ctx &c_) : c(c_) { c.need_lists = true; }
+        ~ListsOn() { c.need_lists = false; }
+    } lists_on(c);
     for(int k = K; k >= 0; --k) {
         if((rc = rollout_recompute(c, k))) return rc;
         if((rc = adjoint_step(c, k > 0))) return rc;
@@ -841,6 +899,11 @@ int sphgpu_get_timers(void *ptr, sphgpu_timers *t)
     t->fluid_force = c.timer_ms[4];
     t->gravity = c.timer_ms[5];
     t->correct = c.timer_ms[6];
+    t->search = c.timer_ms[7];
+    t->pre_pairs = c.timer_ms[8];
+    t->force_pairs = c.timer_ms[9];
+    t->gravity_walk = c.timer_ms[10];
+    t->gravity_leaf = c.timer_ms[11];
     return SPHGPU_OK;
 }
 

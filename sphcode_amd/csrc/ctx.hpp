@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+
+#include <algorithm>
 #include <string>
 
 #include "../../include/sphgpu.h"
@@ -163,7 +165,8 @@ __device__ __forceinline__ int list_slot(int kk, int front, int cap) { return kk
 // Device-side counters (int64), reduced per wave before the atomic.
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
-    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, C_GRAV_REWALK, C_GRAV_SLOTS, NCOUNTERS
+    C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, C_GRAV_REWALK, C_GRAV_SLOTS,
+    C_TILE_OVER, NCOUNTERS
 };
 
 // Device scalars.
@@ -226,7 +229,30 @@ struct Ctx {
     int *num_groups = nullptr;   // device scalar
     int64_t group_cap = 0;
 
-    // neighbour lists (addressing: list_at).  The first
+    // candidate tiles of the last neighbour search, per search group g
+    // (groups[g], at most 64 particles): tile_nc[g] candidates, stored in
+    // the group's slab of tile_wcap slots starting at g * tile_wcap.  With
+    // T = ceil(nc / 64) tiles, staging-order candidate c (the order of the
+    // reference's one-leaf-at-a-time walk) is lane k = c / T of tile
+    // t = c % T: tile t's 64 candidate indices sit at slab + 64 t (-1 past
+    // the end), and lane l of the group has the bit k of tile_mf[slab + 64 t
+    // + l] set when candidate (t, k) is in its front set (the gather
+    // candidates, r < R_l) and of tile_mb[...] when it is one of its other
+    // symmetric candidates.  Consecutive candidates land in different
+    // tiles, so each tile samples the whole neighbourhood of the group and
+    // the group's lanes hold similar numbers of hits in every tile.
+    int *tile_cand = nullptr;
+    uint64_t *tile_mf = nullptr, *tile_mb = nullptr;
+    int *tile_nc = nullptr;
+    int tile_wcap = 0;             // slots per group (multiple of 64)
+    int64_t tile_groups_cap = 0;   // groups the slabs are allocated for
+    int *ws_j = nullptr;           // search waves' staging of kept candidates
+    double *ws_lim = nullptr;
+    int64_t ws_waves = 0;          // search waves the staging is allocated for
+    int ngroups = 0;               // groups of the last tree (host copy)
+
+    // neighbour lists (addressing: list_at), expanded from the tiles when
+    // the adjoint or the set record needs them (need_lists).  The first
     // list_cnt[i] entries (k = 0, 1, ...) are the gather candidates
     // (r < h_guess * ratio) with |r_ij| in list_r / list_rf; the combined
     // search appends the remaining symmetric candidates from the other end
@@ -236,7 +262,8 @@ struct Ctx {
     float *list_rf = nullptr;  // |r_ij| of gather candidates (mixed mode)
     int *list_cnt = nullptr;
     int *list_cnt2 = nullptr;
-    bool lists_symmetric = false;  // the lists hold every force pair
+    bool need_lists = false;       // expand every search's tiles into list
+    bool lists_symmetric = false;  // the tiles hold every force pair
     int64_t trunc_seen = 0;        // truncation counter at the last check
     int *rec_gather = nullptr, *rec_gather_cnt = nullptr;  // recorded sets (debug)
     int *rec_force = nullptr, *rec_force_cnt = nullptr;
@@ -270,8 +297,9 @@ struct Ctx {
 
     // stage timers
     bool timers_on = false;
-    hipEvent_t ev[16];
-    double timer_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool kernel_marks = false;  // sphgpu_integrate with timers on: mark kernel spans
+    hipEvent_t ev[16];          // ev[0..7] stage boundaries, ev[8..13] kernel spans
+    double timer_ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t steps = 0;
     int64_t last_nodes = 0, last_levels = 0;
 };
@@ -300,6 +328,8 @@ int search_gather(Ctx &c, bool initial);  // h from the density guess
 int search_symmetric(Ctx &c);
 int search_guess(Ctx &c);        // sml = h_guess * ratio
 int search_combined(Ctx &c);     // gather and symmetric candidates in one walk
+int search_alloc(Ctx &c);        // tile slabs for c.ngroups groups (after a tree build)
+int search_expand(Ctx &c, int *list, int *cnt, int *cnt2, bool front_only);  // tiles -> list_at lists
 // pre.hip
 int pre_initial(Ctx &c);         // initial_smoothing density
 int pre_compute(Ctx &c);         // Newton h, density, pressure, grad-h, AV switches / MUSCL gradients
@@ -320,6 +350,9 @@ int gravity_combine(Ctx &c, hipStream_t st);
 // persistent launch (slot 0: the neighbour search).
 void reset_work(Ctx &c, int slot);
 constexpr int WORK_SLOTS = 1;
+// one wave per search group: grid of ceil(ngroups / 4) blocks of 4 waves,
+// block b serving groups 4 xcd_block() .. + 3
+inline int group_grid(const Ctx &c) { return std::max(1, (c.ngroups + BLOCK / WAVE - 1) / (BLOCK / WAVE)); }
 // step.hip
 int step_init_fields(Ctx &c);
 int step_timestep(Ctx &c);       // dt into scal[S_DT]

@@ -1,5 +1,8 @@
-// FluidForce on the device, one thread per particle over its symmetric
-// candidate list (pairs with r < max(h_i, h_j) and r != 0):
+// FluidForce on the device, one wave per search group of at most 64
+// particles, one lane per particle, over the group's candidate tiles: the
+// tile's 64 neighbour records are staged once in LDS, and each lane then
+// takes the members of its symmetric candidate set (front | back hit
+// masks) among them, keeping the pairs with r < max(h_i, h_j), r != 0:
 //   SSPH  src/fluid_force.cpp:26-86, viscosity :89-106, conductivity :108-116
 //   DISPH src/disph/d_fluid_force.cpp:26-86 (pressure-energy form)
 //   GSPH  src/gsph/g_fluid_force.cpp:39-166, limiter :27-36, HLL :168-199
@@ -13,9 +16,12 @@
 namespace sphgpu {
 
 struct ForceArgs {
-    const int *list;
-    const int *cnt;
-    const int *cnt2;
+    const int2 *groups;
+    int ngroups;
+    const int *cand;        // candidate tiles (Ctx::tile_cand)
+    const uint64_t *mf, *mb;
+    const int *gnc;
+    int wcap;
     int cap;
     double gamma;
     int use_ac;
@@ -24,7 +30,7 @@ struct ForceArgs {
     int second_order;
     const double *scal;   // S_DT
     int64_t *counters;
-    int *rec;             // recorded force sets (debug), same layout as list
+    int *rec;             // recorded force sets (debug), list_at layout
     int *rec_cnt;
     const double4 *posrec;  // x, y, z, h (FP64)
     const void *frec;       // ForceRec<T>
@@ -114,27 +120,37 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool FP64 = sizeof(T) == 8;
-    const int i = xcd_block() * BLOCK + threadIdx.x;
-    const bool valid = i < n;
+    constexpr bool GS = TYPE == SPHGPU_GSPH;
+    constexpr int SW = BLOCK / WAVE;
+    __shared__ double4 l_p[SW][WAVE];
+    __shared__ ForceRec<T> l_f[SW][WAVE];
+    __shared__ GradRec<T> l_g[GS ? SW : 1][GS ? WAVE : 1];
+    __shared__ int l_j[SW][WAVE];
+    const int w = threadIdx.x / WAVE;
+    const int lane = wave_lane();
+    const int grp = xcd_block() * SW + w;
+    if(grp >= a.ngroups) return;
+    const int2 gr = a.groups[grp];
+    const int i = gr.x + lane;
+    const bool valid = i < gr.y;
     int64_t pairs = 0;
-    if(valid) {
-        const int cnt = a.cnt[i];
-        const int total = cnt + a.cnt2[i];
+    {
         double xi[D];
         T vi[D];
+        const int ii = valid ? i : gr.x;
 #pragma unroll
-        for(int k = 0; k < D; ++k) { xi[k] = s.f[F_X + k][i]; vi[k] = (T)s.f[F_V + k][i]; }
-        const double hi_d = s.f[F_SML][i];
+        for(int k = 0; k < D; ++k) { xi[k] = s.f[F_X + k][ii]; vi[k] = (T)s.f[F_V + k][ii]; }
+        const double hi_d = s.f[F_SML][ii];
         const T hi = (T)hi_d;
-        const Kern ki(hi, (T)s.f[F_HINV][i]);
-        const T mi = (T)s.f[F_MASS][i];
-        const T rhoi = (T)s.f[F_DENS][i];
-        const T pi_ = (T)s.f[F_PRES][i];
-        const T ui = (T)s.f[F_ENE][i];
-        const T ci = (T)s.f[F_SOUND][i];
-        const T alphai = (T)s.f[F_ALPHA][i];
-        const T bali = (T)s.f[F_BAL][i];
-        const T gradhi = TYPE == SPHGPU_GSPH ? T(0) : (T)s.f[F_GRADH][i];
+        const Kern ki(hi, (T)s.f[F_HINV][ii]);
+        const T mi = (T)s.f[F_MASS][ii];
+        const T rhoi = (T)s.f[F_DENS][ii];
+        const T pi_ = (T)s.f[F_PRES][ii];
+        const T ui = (T)s.f[F_ENE][ii];
+        const T ci = (T)s.f[F_SOUND][ii];
+        const T alphai = (T)s.f[F_ALPHA][ii];
+        const T bali = (T)s.f[F_BAL][ii];
+        const T gradhi = TYPE == SPHGPU_GSPH ? T(0) : (T)s.f[F_GRADH][ii];
         const T gm1 = (T)(a.gamma - 1.0);
         const T gam = (T)a.gamma;
         // per-formulation constants of particle i
@@ -144,13 +160,34 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         const T mu_inv_i = T(1) / (mi * ui);
         const T rho2_inv_i = T(1) / (rhoi * rhoi);
         const T dt = (T)a.scal[S_DT];
+        GradRec<T> gri{};
+        if constexpr(GS)
+            if(a.second_order) gri = ((const GradRec<T> *)a.grec)[ii];
 
         Acc<T> acc[D], dene;
         int used = 0;
         const ForceRec<T> *frec = (const ForceRec<T> *)a.frec;
-        for(int kk = 0; kk < total; ++kk) {
-            const int j = a.list[list_at(i, list_slot(kk, cnt, a.cap), a.cap)];
-            const double4 pj4 = a.posrec[j];
+        const size_t gbase = (size_t)grp * a.wcap;
+        const int nc = a.gnc[grp];
+        const int ntile = (nc + WAVE - 1) / WAVE;
+        for(int tt = 0; tt < ntile; ++tt) {
+            const size_t o = gbase + (size_t)tt * WAVE;
+            // stage the tile's records: lane k loads candidate k's
+            const int jl = a.cand[o + lane];
+            if(jl >= 0) {
+                l_p[w][lane] = a.posrec[jl];
+                l_f[w][lane] = frec[jl];
+                if constexpr(GS)
+                    if(a.second_order) l_g[w][lane] = ((const GradRec<T> *)a.grec)[jl];
+            }
+            l_j[w][lane] = jl;
+            uint64_t mm = valid ? (a.mf[o + lane] | a.mb[o + lane]) : 0ull;
+            wave_lds_sync();
+            while(mm) {
+            const int kc = __builtin_ctzll(mm);
+            mm &= mm - 1ull;
+            const int j = l_j[w][kc];
+            const double4 pj4 = l_p[w][kc];
             double xj[D], d[D];
             xj[0] = pj4.x;
             if constexpr(D > 1) xj[1] = pj4.y;
@@ -166,12 +203,12 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 if(!(r2 < hmax * hmax) || r2 == 0.0) continue;
                 r = sqrtf((float)r2);
             }
-            if(a.rec) a.rec[list_at(i, used, a.cap)] = j;
+            if(a.rec && used < a.cap) a.rec[list_at(i, used, a.cap)] = j;
             ++used;
             T rij[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) rij[l] = (T)d[l];
-            const ForceRec<T> fj = frec[j];
+            const ForceRec<T> fj = l_f[w][kc];
             const Kern kj(T(0), fj.hinv);  // the kernel object reads 1/h only
             const T cwi = ki.dw_coef(ki.q_of(r), r);
             const T cwj = kj.dw_coef(kj.q_of(r), r);
@@ -201,8 +238,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     const T di = T(0.5) * (T(1) - ci * dt * rinv);
                     const T dj = T(0.5) * (T(1) - cj * dt * rinv);
                     const T dv = ve_i - ve_j;
-                    const GradRec<T> *grec = (const GradRec<T> *)a.grec;
-                    const GradRec<T> gri = grec[i], grj = grec[j];
+                    const GradRec<T> &grj = l_g[GS ? w : 0][GS ? kc : 0];
                     T gvi[D], gvj[D];
 #pragma unroll
                     for(int c = 0; c < D; ++c) {
@@ -295,12 +331,16 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 for(int l = 0; l < D; ++l) acc[l].add(-(dwi[l] * ci_f + dwj[l] * cj_f));
                 dene.add(de + T(0.5) * mj * pij * vdwij + ac);
             }
+            }
+            wave_lds_sync();  // the next tile overwrites the records
         }
+        if(valid) {
 #pragma unroll
-        for(int k = 0; k < D; ++k) s.f[F_A + k][i] = acc[k].val();
-        s.f[F_DENE][i] = dene.val();
-        if(a.rec_cnt) a.rec_cnt[i] = used;
-        pairs = used;
+            for(int k = 0; k < D; ++k) s.f[F_A + k][i] = acc[k].val();
+            s.f[F_DENE][i] = dene.val();
+            if(a.rec_cnt) a.rec_cnt[i] = used < a.cap ? used : a.cap;
+            pairs = used;
+        }
     }
     pairs = wave_sum(pairs);
     if(wave_lane() == 0 && pairs) atomicAdd((unsigned long long *)&a.counters[C_FORCE_PAIRS], (unsigned long long)pairs);
@@ -312,7 +352,7 @@ template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, cons
     hipLaunchKernelGGL((pack_force_kernel<D, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, use_u,
                        c.posrec, (ForceRec<T> *)c.pair_rec, (GradRec<T> *)c.grad_rec);
     if(c.after_pack) c.after_pack_rc = c.after_pack(c);
-    hipLaunchKernelGGL((force_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((force_kernel<D, K, TYPE, T>), dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
 template <int D, int K, class T> static void launch_force_t(Ctx &c, const ForceArgs &a, const BoxDev &box)
@@ -337,9 +377,13 @@ template <class T> static void force_t(Ctx &c, const ForceArgs &a, const BoxDev 
 int force_compute(Ctx &c)
 {
     ForceArgs a{};
-    a.list = c.list;
-    a.cnt = c.list_cnt;
-    a.cnt2 = c.list_cnt2;
+    a.groups = c.groups;
+    a.ngroups = c.ngroups;
+    a.cand = c.tile_cand;
+    a.mf = c.tile_mf;
+    a.mb = c.tile_mb;
+    a.gnc = c.tile_nc;
+    a.wcap = c.tile_wcap;
     a.cap = c.cap;
     a.gamma = c.p.gamma;
     a.use_ac = c.p.ac_is_valid;

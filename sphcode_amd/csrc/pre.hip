@@ -1,5 +1,7 @@
-// PreInteraction on the device, one thread per particle over its gather
-// candidate list:
+// PreInteraction on the device, one wave per search group of at most 64
+// particles, one lane per particle, over the group's candidate tiles: each
+// tile's 64 neighbour records are staged once in LDS and every lane takes
+// the members of its gather candidate set (front hit mask) among them:
 //   SSPH  src/pre_interaction.cpp:39-169 (Newton :227-283)
 //   DISPH src/disph/d_pre_interaction.cpp:21-162 (Newton :174-230)
 //   GSPH  src/gsph/g_pre_interaction.cpp:27-144
@@ -27,11 +29,12 @@
 namespace sphgpu {
 
 struct PreArgs {
-    const int *list;
-    const double *list_r;
-    const float *list_rf;
-    const int *cnt;
-    int cap;
+    const int2 *groups;
+    int ngroups;
+    const int *cand;      // candidate tiles (Ctx::tile_cand)
+    const uint64_t *mf;   // front (gather candidate) masks
+    const int *gnc;
+    int wcap;
     double gamma;
     double nngb;
     double ratio;
@@ -47,6 +50,16 @@ struct PreArgs {
     void *gpart;          // GravPart<T> per particle, written with h (gravity on)
 };
 
+constexpr int PSW = BLOCK / WAVE;  // waves (groups) per block
+
+// |r_ij| of a gather candidate in the pair type from the FP64 r^2: the
+// correctly rounded FP64 square root, or its FP32 form (gather_r_f32)
+template <class T> __device__ __forceinline__ T gather_r(double r2)
+{
+    if constexpr(sizeof(T) == 8) return sqrt(r2);
+    else return gather_r_f32(r2);
+}
+
 // Records read for j by the final sweep (FP32 mirrors in mixed mode).
 template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_pre_kernel(State s, int n, PreRec<T> *out)
 {
@@ -61,12 +74,6 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_pre_kern
 #pragma unroll
     for(int k = 0; k < D; ++k) r.v[k] = fld<T>(s, F_V + k, i);
     out[i] = r;
-}
-
-template <class T> __device__ __forceinline__ T list_r(const PreArgs &a, size_t k)
-{
-    if constexpr(sizeof(T) == 8) return a.list_r[k];
-    else return a.list_rf[k];
 }
 
 // Writes an FP64 result and, in mixed mode, its FP32 mirror.
@@ -98,21 +105,54 @@ template <int D, class T> struct CurlAcc {
 
 // Initial smoothing (src/pre_interaction.cpp:171-215): density with
 // h = h_guess over candidates with r < h.
+template <class T> struct PosMass {
+    double x[3];
+    T m;
+};
+
 template <int D, int K, class T>
-__global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, int n, PreArgs a)
+__global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box, PreArgs a)
 {
     using Kern = typename KernelOf<D, K, T>::type;
-    const int i = xcd_block() * BLOCK + threadIdx.x;
-    if(i >= n) return;
-    const int cnt = a.cnt[i];
-    const T h = (T)s.f[F_SML][i];
+    __shared__ PosMass<T> l_r[PSW][WAVE];
+    const int w = threadIdx.x / WAVE;
+    const int lane = wave_lane();
+    const int grp = xcd_block() * PSW + w;
+    if(grp >= a.ngroups) return;
+    const int2 gr = a.groups[grp];
+    const int i = gr.x + lane;
+    const bool valid = i < gr.y;
+    const int ii = valid ? i : gr.x;
+    double xi[D];
+#pragma unroll
+    for(int k = 0; k < D; ++k) xi[k] = s.f[F_X + k][ii];
+    const T h = (T)s.f[F_SML][ii];
     const Kern ker(h);
     Acc<T> rho;
-    for(int k = 0; k < cnt; ++k) {
-        const T r = list_r<T>(a, list_at(i, k, a.cap));
-        if(r < h) rho.add(fld<T>(s, F_MASS, a.list[list_at(i, k, a.cap)]) * ker.w(ker.q_of(r)));
+    const size_t gbase = (size_t)grp * a.wcap;
+    const int ntile = (a.gnc[grp] + WAVE - 1) / WAVE;
+    for(int tt = 0; tt < ntile; ++tt) {
+        const size_t o = gbase + (size_t)tt * WAVE;
+        const int jl = a.cand[o + lane];
+        if(jl >= 0) {
+#pragma unroll
+            for(int k = 0; k < D; ++k) l_r[w][lane].x[k] = s.f[F_X + k][jl];
+            l_r[w][lane].m = fld<T>(s, F_MASS, jl);
+        }
+        uint64_t mm = valid ? a.mf[o + lane] : 0ull;
+        wave_lds_sync();
+        while(mm) {
+            const int kc = __builtin_ctzll(mm);
+            mm &= mm - 1ull;
+            double xj[D], d[D];
+#pragma unroll
+            for(int k = 0; k < D; ++k) xj[k] = l_r[w][kc].x[k];
+            const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
+            if(r < h) rho.add(l_r[w][kc].m * ker.w(ker.q_of(r)));
+        }
+        wave_lds_sync();
     }
-    s.f[F_DENS][i] = rho.val();
+    if(valid) s.f[F_DENS][i] = rho.val();
 }
 
 template <int D, int K, int TYPE, class T>
@@ -120,61 +160,95 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool NUMBER = TYPE == SPHGPU_DISPH;
-    const int i = xcd_block() * BLOCK + threadIdx.x;
-    const bool valid = i < n;
+    __shared__ PreRec<T> l_r[PSW][WAVE];
+    __shared__ int l_j[PSW][WAVE];
+    const int w = threadIdx.x / WAVE;
+    const int lane = wave_lane();
+    // waves past the last group take part in the block's reduction only
+    const int grp = xcd_block() * PSW + w;
+    const bool have = grp < a.ngroups;
+    const int2 gr = have ? a.groups[grp] : make_int2(0, 0);
+    const int i = gr.x + lane;
+    const bool valid = have && i < gr.y;
+    const int ii = valid ? i : gr.x;
+    const size_t gbase = (size_t)(have ? grp : 0) * a.wcap;
+    const int ntile = have ? (a.gnc[grp] + WAVE - 1) / WAVE : 0;
+    const PreRec<T> *prec = (const PreRec<T> *)a.prec;
     double hvs = 1.7976931348623157e308;
     int64_t iters = 0, fails = 0, trunc = 0, pairs = 0;
-    if(valid) {
-        const int cnt = a.cnt[i];
-        const double mi = s.f[F_MASS][i];
-        const double Hs = s.f[F_SML][i];  // search radius h_guess * ratio
+    if(have) {
+        double xi[D];
+#pragma unroll
+        for(int k = 0; k < D; ++k) xi[k] = s.f[F_X + k][ii];
+        const double mi = s.f[F_MASS][ii];
+        const double Hs = s.f[F_SML][ii];  // search radius h_guess * ratio
         double h = Hs;
         if(a.iterative) {
-            // PreInteraction::newton_raphson
+            // PreInteraction::newton_raphson, every lane on its own iterate;
+            // the wave sweeps the tiles once per iteration for the lanes
+            // still iterating
             const double b = NUMBER ? a.nngb / UnitVolume<D>::value : mi * a.nngb / UnitVolume<D>::value;
             double hn = Hs / a.ratio;
-            bool done = false;
-            for(int it = 0; it < 10 && !done; ++it) {
-                ++iters;
+            bool done = !valid;
+            int it = 0;
+            for(; it < 10; ++it) {
+                if(uniform64(__ballot(!done)) == 0) break;
                 const double hb = hn;
                 const T ht = (T)hn;
                 const Kern ker(ht);
                 Acc<T> rho, drho;
-                for(int k = 0; k < cnt; ++k) {
-                    const T r = list_r<T>(a, list_at(i, k, a.cap));
-                    if(r < ht) {
-                        const T q = ker.q_of(r);
-                        if constexpr(NUMBER) {
-                            rho.add(ker.w(q));
-                            drho.add(ker.dhw(q));
-                        } else {
-                            const T mj = fld<T>(s, F_MASS, a.list[list_at(i, k, a.cap)]);
-                            rho.add(mj * ker.w(q));
-                            drho.add(mj * ker.dhw(q));
+                for(int tt = 0; tt < ntile; ++tt) {
+                    const size_t o = gbase + (size_t)tt * WAVE;
+                    const int jl = a.cand[o + lane];
+                    if(jl >= 0) l_r[w][lane] = prec[jl];
+                    uint64_t mm = done ? 0ull : a.mf[o + lane];
+                    wave_lds_sync();
+                    while(mm) {
+                        const int kc = __builtin_ctzll(mm);
+                        mm &= mm - 1ull;
+                        double xj[D], d[D];
+#pragma unroll
+                        for(int k = 0; k < D; ++k) xj[k] = l_r[w][kc].x[k];
+                        const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
+                        if(r < ht) {
+                            const T q = ker.q_of(r);
+                            if constexpr(NUMBER) {
+                                rho.add(ker.w(q));
+                                drho.add(ker.dhw(q));
+                            } else {
+                                const T mj = l_r[w][kc].m;
+                                rho.add(mj * ker.w(q));
+                                drho.add(mj * ker.dhw(q));
+                            }
                         }
                     }
+                    wave_lds_sync();
                 }
-                const double f = rho.val() * pow_dim<D>(hn) - b;
-                const double df = drho.val() * pow_dim<D>(hn) + D * rho.val() * pow_dim1<D>(hn);
-                hn -= f / df;
-                if(fabs(hn - hb) < (hn + hb) * 1e-4) done = true;
+                if(!done) {
+                    ++iters;
+                    const double f = rho.val() * pow_dim<D>(hn) - b;
+                    const double df = drho.val() * pow_dim<D>(hn) + D * rho.val() * pow_dim1<D>(hn);
+                    hn -= f / df;
+                    if(fabs(hn - hb) < (hn + hb) * 1e-4) done = true;
+                }
             }
-            if(done) {
-                h = hn;
-            } else {
-                h = Hs / a.ratio;
-                ++fails;
+            if(valid) {
+                if(done) {
+                    h = hn;
+                } else {
+                    h = Hs / a.ratio;
+                    ++fails;
+                }
+                if(h > Hs) ++trunc;
             }
-            if(h > Hs) ++trunc;
         }
 
         // final sums over r < h
-        double xi[D];
         T vi[D];
 #pragma unroll
-        for(int k = 0; k < D; ++k) { xi[k] = s.f[F_X + k][i]; vi[k] = (T)s.f[F_V + k][i]; }
-        const double ui = s.f[F_ENE][i];
-        const double ci = s.f[F_SOUND][i];
+        for(int k = 0; k < D; ++k) vi[k] = (T)s.f[F_V + k][ii];
+        const double ui = s.f[F_ENE][ii];
+        const double ci = s.f[F_SOUND][ii];
         const T ht = (T)h;
         const Kern ker(ht);
         Acc<T> rho, drho, nd, dnd, pb, dpb, divv;
@@ -184,41 +258,46 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
         const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
-        const PreRec<T> *prec = (const PreRec<T> *)a.prec;
-        for(int k = 0; k < cnt; ++k) {
-            const T r = list_r<T>(a, list_at(i, k, a.cap));
+        for(int tt = 0; tt < ntile; ++tt) {
+            const size_t o = gbase + (size_t)tt * WAVE;
+            const int jl = a.cand[o + lane];
+            if(jl >= 0) l_r[w][lane] = prec[jl];
+            l_j[w][lane] = jl;
+            uint64_t mm = valid ? a.mf[o + lane] : 0ull;
+            wave_lds_sync();
+            while(mm) {
+            const int kc = __builtin_ctzll(mm);
+            mm &= mm - 1ull;
+            const PreRec<T> pj = l_r[w][kc];
+            double xj[D], d[D];
+#pragma unroll
+            for(int l = 0; l < D; ++l) xj[l] = pj.x[l];
+            const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
             if(!(r < ht)) continue;
-            const int j = a.list[list_at(i, k, a.cap)];
+            const int j = l_j[w][kc];
             ++count;
-            const PreRec<T> pj = prec[j];
             const T mj = pj.m;
             const T q = ker.q_of(r);
-            const T w = ker.w(q);
+            const T wk = ker.w(q);
             if constexpr(TYPE == SPHGPU_DISPH) {
                 const T dh = ker.dhw(q);
                 const T uj = pj.u;
-                rho.add(mj * w);
-                nd.add(w);
-                pb.add(mj * uj * w);
+                rho.add(mj * wk);
+                nd.add(wk);
+                pb.add(mj * uj * wk);
                 dpb.add(mj * uj * dh);
                 dnd.add(dh);
             } else if constexpr(TYPE == SPHGPU_SSPH) {
-                rho.add(mj * w);
+                rho.add(mj * wk);
                 drho.add(mj * ker.dhw(q));
             } else {
-                rho.add(mj * w);
+                rho.add(mj * wk);
             }
             T rij[D], vij[D];
-            if(j != i || need_dv) {
-                double xj[D], d[D];
 #pragma unroll
-                for(int l = 0; l < D; ++l) xj[l] = pj.x[l];
-                sep2<D>(box, xi, xj, d);
-#pragma unroll
-                for(int l = 0; l < D; ++l) {
-                    rij[l] = (T)d[l];
-                    vij[l] = vi[l] - pj.v[l];
-                }
+            for(int l = 0; l < D; ++l) {
+                rij[l] = (T)d[l];
+                vij[l] = vi[l] - pj.v[l];
             }
             if(j != i) {
                 T rv = rij[0] * vij[0];
@@ -254,74 +333,78 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                     rot.add(vij, dw, wgt);
                 }
             }
-        }
-        pairs = count;
-        const double gm1 = a.gamma - 1.0;
-        const double rho_d = rho.val();
-        put<T>(s, F_SML, i, h);
-        put<T>(s, F_HINV, i, 1.0 / h);
-        if(a.gpart) {
-            // the gravity leaf pass's record of i, as pack_parts_kernel writes it
-            GravPart<T> gp{};
-#pragma unroll
-            for(int k = 0; k < 3; ++k) gp.x[k] = k < D ? xi[k] : 0.0;
-            gp.m = (T)mi;
-            gp.einv = T(2) * (T)(1.0 / h);  // 2 x the FP32 mirror of 1/h in mixed mode
-            ((GravPart<T> *)a.gpart)[i] = gp;
-        }
-        put<T>(s, F_DENS, i, rho_d);
-        double pres;
-        if constexpr(TYPE == SPHGPU_DISPH) {
-            pres = gm1 * pb.val();
-            const double nv = nd.val();
-            put<T>(s, F_GRADH, i, h / (D * nv) * dpb.val() / (1.0 + h / (D * nv) * dnd.val()));
-        } else {
-            pres = gm1 * rho_d * ui;
-            if constexpr(TYPE == SPHGPU_SSPH) put<T>(s, F_GRADH, i, 1.0 / (1.0 + h / (D * rho_d) * drho.val()));
-        }
-        put<T>(s, F_PRES, i, pres);
-        s.nb[i] = count;
-        hvs = h / (double)vsig_max;
-
-        if constexpr(TYPE == SPHGPU_GSPH) {
-            if(a.second_order) {
-                const double rinv = 1.0 / rho_d;
-#pragma unroll
-                for(int l = 0; l < D; ++l) {
-                    put<T>(s, F_GD + l, i, gd[l].val());
-                    put<T>(s, F_GP + l, i, (gd[l].val() * ui + gu[l].val()) * gm1);
-                }
-#pragma unroll
-                for(int c = 0; c < D; ++c)
-#pragma unroll
-                    for(int l = 0; l < D; ++l) put<T>(s, F_GV + 3 * c + l, i, gv[c][l].val() * rinv);
             }
-        } else {
-            const double dt = a.scal[S_DT];
-            const double alpha = s.f[F_ALPHA][i];
-            double dv = divv.val();
-            if(a.balsara && D != 1) {
-                double rm;
-                if constexpr(TYPE == SPHGPU_DISPH) {
-                    const double p_inv = gm1 / pres;
-                    dv *= p_inv;
-                    rm = rot.magnitude() * fabs(p_inv);
-                } else {
-                    dv /= rho_d;
-                    rm = rot.magnitude() / rho_d;
+            wave_lds_sync();  // the next tile overwrites the records
+        }
+        if(valid) {
+            pairs = count;
+            const double gm1 = a.gamma - 1.0;
+            const double rho_d = rho.val();
+            put<T>(s, F_SML, i, h);
+            put<T>(s, F_HINV, i, 1.0 / h);
+            if(a.gpart) {
+                // the gravity leaf pass's record of i, as pack_parts_kernel writes it
+                GravPart<T> gp{};
+#pragma unroll
+                for(int k = 0; k < 3; ++k) gp.x[k] = k < D ? xi[k] : 0.0;
+                gp.m = (T)mi;
+                gp.einv = T(2) * (T)(1.0 / h);  // 2 x the FP32 mirror of 1/h in mixed mode
+                ((GravPart<T> *)a.gpart)[i] = gp;
+            }
+            put<T>(s, F_DENS, i, rho_d);
+            double pres;
+            if constexpr(TYPE == SPHGPU_DISPH) {
+                pres = gm1 * pb.val();
+                const double nv = nd.val();
+                put<T>(s, F_GRADH, i, h / (D * nv) * dpb.val() / (1.0 + h / (D * nv) * dnd.val()));
+            } else {
+                pres = gm1 * rho_d * ui;
+                if constexpr(TYPE == SPHGPU_SSPH) put<T>(s, F_GRADH, i, 1.0 / (1.0 + h / (D * rho_d) * drho.val()));
+            }
+            put<T>(s, F_PRES, i, pres);
+            s.nb[i] = count;
+            hvs = h / (double)vsig_max;
+
+            if constexpr(TYPE == SPHGPU_GSPH) {
+                if(a.second_order) {
+                    const double rinv = 1.0 / rho_d;
+#pragma unroll
+                    for(int l = 0; l < D; ++l) {
+                        put<T>(s, F_GD + l, i, gd[l].val());
+                        put<T>(s, F_GP + l, i, (gd[l].val() * ui + gu[l].val()) * gm1);
+                    }
+#pragma unroll
+                    for(int c = 0; c < D; ++c)
+#pragma unroll
+                        for(int l = 0; l < D; ++l) put<T>(s, F_GV + 3 * c + l, i, gv[c][l].val() * rinv);
                 }
-                put<T>(s, F_BAL, i, fabs(dv) / (fabs(dv) + rm + 1e-4 * ci / h));
-                if(a.tdav) {
+            } else {
+                const double dt = a.scal[S_DT];
+                const double alpha = s.f[F_ALPHA][i];
+                double dv = divv.val();
+                if(a.balsara && D != 1) {
+                    double rm;
+                    if constexpr(TYPE == SPHGPU_DISPH) {
+                        const double p_inv = gm1 / pres;
+                        dv *= p_inv;
+                        rm = rot.magnitude() * fabs(p_inv);
+                    } else {
+                        dv /= rho_d;
+                        rm = rot.magnitude() / rho_d;
+                    }
+                    put<T>(s, F_BAL, i, fabs(dv) / (fabs(dv) + rm + 1e-4 * ci / h));
+                    if(a.tdav) {
+                        const double tau_inv = a.eps_av * ci / h;
+                        const double da = (-(alpha - a.amin) * tau_inv + fmax(-dv, 0.0) * (a.amax - alpha)) * dt;
+                        put<T>(s, F_ALPHA, i, alpha + da);
+                    }
+                } else if(a.tdav) {
+                    if constexpr(TYPE == SPHGPU_DISPH) dv *= gm1 / pres;
+                    else dv /= rho_d;
                     const double tau_inv = a.eps_av * ci / h;
-                    const double da = (-(alpha - a.amin) * tau_inv + fmax(-dv, 0.0) * (a.amax - alpha)) * dt;
-                    put<T>(s, F_ALPHA, i, alpha + da);
+                    const double si = fmax(-dv, 0.0);
+                    put<T>(s, F_ALPHA, i, (alpha + dt * tau_inv * a.amin + si * dt * a.amax) / (1.0 + dt * tau_inv + si * dt));
                 }
-            } else if(a.tdav) {
-                if constexpr(TYPE == SPHGPU_DISPH) dv *= gm1 / pres;
-                else dv /= rho_d;
-                const double tau_inv = a.eps_av * ci / h;
-                const double si = fmax(-dv, 0.0);
-                put<T>(s, F_ALPHA, i, (alpha + dt * tau_inv * a.amin + si * dt * a.amax) / (1.0 + dt * tau_inv + si * dt));
             }
         }
     }
@@ -342,7 +425,7 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
     __syncthreads();
     if(threadIdx.x == 0) {
         double m = sh[0];
-        for(int w = 1; w < BLOCK / WAVE; ++w) m = sh[w] < m ? sh[w] : m;
+        for(int ww = 1; ww < BLOCK / WAVE; ++ww) m = sh[ww] < m ? sh[ww] : m;
         a.partial[blockIdx.x] = m;
     }
 }
@@ -350,11 +433,12 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 static PreArgs make_args(Ctx &c)
 {
     PreArgs a{};
-    a.list = c.list;
-    a.list_r = c.list_r;
-    a.list_rf = c.list_rf;
-    a.cnt = c.list_cnt;
-    a.cap = c.cap;
+    a.groups = c.groups;
+    a.ngroups = c.ngroups;
+    a.cand = c.tile_cand;
+    a.mf = c.tile_mf;
+    a.gnc = c.tile_nc;
+    a.wcap = c.tile_wcap;
     a.gamma = c.p.gamma;
     a.nngb = (double)c.p.neighbor_number;
     a.ratio = c.p.iterative_sml ? 1.2 : 1.0;
@@ -380,7 +464,7 @@ constexpr int kNumPreInputs = 8;
 
 template <int D, int K, class T> static void launch_initial(Ctx &c, const PreArgs &a)
 {
-    hipLaunchKernelGGL((pre_initial_kernel<D, K, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n, a);
+    hipLaunchKernelGGL((pre_initial_kernel<D, K, T>), dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.s, make_box(c), a);
 }
 
 template <class T> static void initial_t(Ctx &c, const PreArgs &a)
@@ -408,7 +492,7 @@ template <int D, int K, int TYPE, class T> static void launch_pre(Ctx &c, const 
 {
     hipLaunchKernelGGL((pack_pre_kernel<D, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n,
                        (PreRec<T> *)c.pair_rec);
-    hipLaunchKernelGGL((pre_kernel<D, K, TYPE, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
+    hipLaunchKernelGGL((pre_kernel<D, K, TYPE, T>), dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 
 template <int D, int K, class T> static void launch_pre_t(Ctx &c, const PreArgs &a, const BoxDev &box)
@@ -441,7 +525,7 @@ int pre_compute(Ctx &c)
     else pre_t<double>(c, a, box);
     SPHGPU_HIP(hipGetLastError());
     c.gpart_fresh = a.gpart != nullptr;
-    return reduce_min_partials(c, grid_for(c.n), S_HVS);
+    return reduce_min_partials(c, group_grid(c), S_HVS);
 }
 
 }  // namespace sphgpu

@@ -1,5 +1,5 @@
 // Neighbour search (BHTree::neighbor_search, src/bhtree.cpp:114-126,
-// 234-270) for 64 Morton-consecutive particles per wave, in two phases:
+// 234-270) for 64 Morton-consecutive particles per wave, in three phases:
 //
 //  1. Group walk.  The wave walks the pre-order tree once, 64 consecutive
 //     nodes at a time (one per lane, visited ones found by a prefix maximum
@@ -10,38 +10,42 @@
 //     search radius (symmetric search: max(that, cell's largest h)).  This
 //     is implied by the reference's per-particle test, so no candidate of
 //     any lane is lost.  Leaves are collected into LDS.
-//  2. Candidate tiles.  The collected leaves' particles (contiguous ranges
-//     in key order) are staged 64 at a time into LDS with coalesced loads;
-//     every lane then applies the reference's leaf test to each staged
-//     particle: |x_i - x_j|^2 < H_i^2 with H_i = h_guess * kernel_ratio
-//     (gather, src/pre_interaction.cpp:61-70) or max(h_i, leaf's largest h)
-//     (symmetric, src/bhtree.cpp:237, 256), into a 64-bit hit mask, and
-//     appends its hits in staging order.
+//  2. Candidate staging.  The collected leaves' particles (contiguous
+//     ranges in key order) are read 64 at a time with coalesced loads; those
+//     within the group's largest radius of the group box are kept, in
+//     staging order, in the wave's staging buffer.
+//  3. Candidate tiles.  With nc kept candidates and T = ceil(nc / 64)
+//     tiles, tile t takes candidates t, t + T, t + 2T, ... (lane k of the
+//     tile holds candidate t + kT).  Every lane applies the reference's
+//     leaf test to each of the tile's candidates: |x_i - x_j|^2 < H_i^2
+//     with H_i = h_guess * kernel_ratio (gather, src/pre_interaction.cpp:
+//     61-70) or max(h_i, leaf's largest h) (symmetric, src/bhtree.cpp:237,
+//     256), into a 64-bit hit mask, written with the tile's candidate
+//     indices to the group's slab (layout: Ctx::tile_cand).
 //
 // The sets are the reference's; the reference then sorts each list by
 // distance (src/bhtree.cpp:120-124) and stops at the first r >= h, whereas
-// the pair loops here test r < h on the unsorted list -- the same set.
+// the pair passes here test r < h on the unsorted set -- the same set.
 //
 // The combined search (split mode) walks with sml = the gather radius and
 // the cells' largest gather radius, keeps pairs with r < max(sml_i, sml_j),
-// and files those with r^2 < sml_i^2 (the gather candidates) at the front of
-// the list and the rest at the back, so that one walk serves both the
+// and marks those with r^2 < sml_i^2 (the gather candidates) in the front
+// mask and the rest in the back mask, so that one walk serves both the
 // pre-interaction and the force stage.
-//
-// List layout: list_at (ctx.hpp).
 #include "ctx.hpp"
 
 namespace sphgpu {
 
 struct SearchArgs {
-    int *list;
-    double *list_r;      // |r_ij| of each gather candidate (FP64 mode)
-    float *list_rf;      // the same in FP32 (mixed mode)
-    int *cnt;            // entries at the front of the list
-    int *cnt2;           // entries at the back of the list
+    int *cand;           // group slabs of candidate tiles (Ctx::tile_cand)
+    uint64_t *mf, *mb;   // front / back hit masks per (tile, lane)
+    int *gnc;            // candidates per group
+    int wcap;            // slots per group slab and per staging buffer
+    int *ws_j;           // staging buffers, one per persistent wave
+    double *ws_lim;      // IJ: the staged candidate's radius squared
     int split;           // symmetric search that also marks the gather set:
-                         // r < sml_i at the front, the rest at the back
-    int cap;
+                         // r < sml_i in the front mask, the rest in the back
+    int cap;             // per-particle capacity (the reference's list size)
     int guess;           // 1: set sml = h_guess * ratio first
     double ratio;
     double nngb;
@@ -51,14 +55,6 @@ struct SearchArgs {
 
 constexpr int SW = BLOCK / WAVE;  // waves per block
 constexpr int LEAFCAP = 128;      // leaves collected per flush
-constexpr int FLATCAP = 256;      // listed (lane, candidate) hits per tile
-
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
-{
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, WAVE);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, WAVE);
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // Wave-wide exclusive prefix sum.
 __device__ __forceinline__ int wave_exclusive_scan(int v, int &total)
@@ -95,8 +91,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     __shared__ double l_ks[SW][LEAFCAP];
     __shared__ double c_x[SW][3][WAVE];
     __shared__ double c_lim[SW][WAVE];  // IJ: the candidate's radius squared
-    __shared__ int c_j[SW][WAVE];
-    __shared__ uint16_t flat[SW][FLATCAP];
 
     const int w = uniform(threadIdx.x / WAVE);
     const int lane = wave_lane();
@@ -125,7 +119,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     }
     const double h2_own = h * h;
     const int64_t h2_bits = __double_as_longlong(h2_own);
-    const size_t lbase = list_base(valid ? i : 0, a.cap);
+    // this wave's staging buffer, and the group's slab of tiles
+    const size_t wbase = (size_t)(blockIdx.x * SW + w) * (size_t)a.wcap;
+    const size_t gbase = (size_t)grp * (size_t)a.wcap;
 
     // group box: centre, half extent, largest radius (wave-uniform)
     double gc[D], ge[D];
@@ -138,11 +134,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     }
     const double hmax = uniform_d(wave_maxv(valid ? h : 0.0));
 
-    int *out = a.list;
-    double *outr = a.list_r;
-    float *outrf = a.list_rf;
-    int count = 0, count2 = 0;  // candidates found (front / back)
-    int wf = 0, wb = 0;          // of those, written (at most cap in total)
+    int count = 0, count2 = 0;  // hits (front / back)
+    int nkept = 0;               // staged candidates (may pass wcap: overflow)
 
     const int total = uniform(*t.num_nodes);
     int node = 0;
@@ -218,7 +211,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         if(lane == 0) l_pre[w][nleaf] = ncand;
         wave_lds_sync();
 
-        // ---- phase 2: candidate tiles of 64
+        // ---- phase 2: stage the collected leaves' particles, 64 at a time
         int lc = 0;  // last leaf starting at or before tb
         for(int tb = 0; tb < ncand; tb += WAVE) {
             // stage: each lane fetches one candidate and keeps it only if it
@@ -256,158 +249,94 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             }
             const uint64_t kept = __ballot(keep);
             if(keep) {
-                const int pos = __popcll(kept & ((1ull << lane) - 1ull));
-                c_j[w][pos] = j;
-#pragma unroll
-                for(int k = 0; k < D; ++k) c_x[w][k][pos] = xj[k];
-                if constexpr(IJ) {
-                    // split mode: r < max(sml_i, sml_j) (each radius bounds
-                    // that particle's final h unless truncated); otherwise
-                    // the reference's r < max(h_i, leaf's largest h).  As
-                    // squares: max(a, b)^2 == max(a^2, b^2) for a, b >= 0.
-                    const double kl = a.split ? s.f[F_SML][j] : ksj;
-                    c_lim[w][pos] = kl * kl;
+                const int at = nkept + __popcll(kept & ((1ull << lane) - 1ull));
+                if(at < a.wcap) {
+                    a.ws_j[wbase + at] = j;
+                    if constexpr(IJ) {
+                        // split mode: r < max(sml_i, sml_j) (each radius bounds
+                        // that particle's final h unless truncated); otherwise
+                        // the reference's r < max(h_i, leaf's largest h).  As
+                        // squares: max(a, b)^2 == max(a^2, b^2) for a, b >= 0.
+                        const double kl = a.split ? s.f[F_SML][j] : ksj;
+                        a.ws_lim[wbase + at] = kl * kl;
+                    }
                 }
             }
             const int m = uniform(__popcll(kept));
+            nkept += m;
             tested += m;
-            wave_lds_sync();
-            // pass A: every lane tests every staged candidate (uniform
-            // loop, broadcast LDS reads) into a hit mask, and a mask of the
-            // hits inside its own radius (the gather entries of a split
-            // search; every hit otherwise)
-            uint64_t mine = 0, mfront = 0;
-            if(valid) {
-                uint32_t mlo = 0, mhi = 0, flo = 0, fhi = 0;
-                auto test = [&](int c, uint32_t &hb, uint32_t &fb) {
-                    double xj[D], d[D];
-#pragma unroll
-                    for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
-                    const double r2 = sep2<D>(box, xi, xj, d);
-                    // r2 < max(h_i^2, kl^2) as two tests; all three are
-                    // non-negative, so their bit patterns order as integers
-                    const int64_t rb = __double_as_longlong(r2);
-                    const bool f = rb < h2_bits;
-                    bool h = f;
-                    if constexpr(IJ) h = f | (rb < __double_as_longlong(c_lim[w][c]));
-                    hb = (uint32_t)h;
-                    fb = (uint32_t)f;
-                };
-                const int m1 = m < 32 ? m : 32;
-                for(int c = 0; c < m1; ++c) {
-                    uint32_t hb, fb;
-                    test(c, hb, fb);
-                    mlo |= hb << c;
-                    flo |= fb << c;
-                }
-                for(int c = 32; c < m; ++c) {
-                    uint32_t hb, fb;
-                    test(c, hb, fb);
-                    mhi |= hb << (c - 32);
-                    fhi |= fb << (c - 32);
-                }
-                mine = ((uint64_t)mhi << 32) | mlo;
-                mfront = a.split ? ((uint64_t)fhi << 32) | flo : mine;
-            }
-            // pass B: append each lane's hits in staging order, gather
-            // entries from the front of its list and force-only entries
-            // from the back.  The tile's (lane, candidate) hits are listed
-            // in LDS and written 64 at a time, one per lane, each at the
-            // slot its rank among its lane's hits gives; a tile in which
-            // some list would reach its capacity (or with more than FLATCAP
-            // hits) takes the one-lane-at-a-time loop, which keeps the
-            // first cap entries.
-            const int nh = __popcll(mine);
-            const int nf = __popcll(mfront);
-            int htotal = 0;
-            const int hbase = wave_exclusive_scan(nh, htotal);
-            htotal = uniform(htotal);
-            const bool fits = wf + wb + nh <= a.cap;
-            if(uniform64(__ballot(!fits)) == 0 && htotal <= FLATCAP) {
-                {
-                    uint64_t mm = mine;
-                    int k = hbase;
-                    while(mm) {
-                        const int c = __builtin_ctzll(mm);
-                        mm &= mm - 1ull;
-                        flat[w][k++] = (uint16_t)((lane << 8) | c);
-                    }
-                }
-                wave_lds_sync();
-                for(int r = 0; r < htotal; r += WAVE) {
-                    const int p = r + lane;
-                    const bool act = p < htotal;
-                    const int code = act ? (int)flat[w][p] : 0;
-                    const int owner = code >> 8, c = code & 255;
-                    const uint64_t om = shfl64(mine, owner), of = shfl64(mfront, owner);
-                    const int owf = __shfl(wf, owner, WAVE), owb = __shfl(wb, owner, WAVE);
-                    double xo[D];
-#pragma unroll
-                    for(int k = 0; k < D; ++k) xo[k] = __longlong_as_double((long long)shfl64((uint64_t)__double_as_longlong(xi[k]), owner));
-                    if(act) {
-                        const uint64_t below = (1ull << c) - 1ull;
-                        const int jj = c_j[w][c];
-                        const int io = gr.x + owner;
-                        const size_t ob = list_base(io, a.cap);
-                        if((of >> c) & 1ull) {
-                            const size_t at = ob + list_off(owf + __popcll(of & below));
-                            out[at] = jj;
-                            if(outr || outrf) {
-                                double xj[D], d[D];
-#pragma unroll
-                                for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
-                                const double r2 = sep2<D>(box, xo, xj, d);
-                                if(outr) outr[at] = sqrt(r2);
-                                if(outrf) outrf[at] = gather_r_f32(r2);
-                            }
-                        } else {
-                            out[ob + list_off(a.cap - 1 - (owb + __popcll(om & ~of & below)))] = jj;
-                        }
-                    }
-                }
-                wf += nf;
-                wb += nh - nf;
-                count += nf;
-                count2 += nh - nf;
-            } else if(valid) {
-                uint64_t mm = mine;
-                while(mm) {
-                    const int c = __builtin_ctzll(mm);
-                    mm &= mm - 1ull;
-                    double xj[D], d[D];
-#pragma unroll
-                    for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][c];
-                    const double r2 = sep2<D>(box, xi, xj, d);
-                    const int jj = c_j[w][c];
-                    const bool room = wf + wb < a.cap;
-                    if(!a.split || r2 < h2_own) {
-                        if(room) {
-                            const size_t at = lbase + list_off(wf);
-                            out[at] = jj;
-                            if(outr) outr[at] = sqrt(r2);
-                            if(outrf) outrf[at] = gather_r_f32(r2);
-                            ++wf;
-                        }
-                        ++count;
-                    } else {
-                        if(room) {
-                            out[lbase + list_off(a.cap - 1 - wb)] = jj;
-                            ++wb;
-                        }
-                        ++count2;
-                    }
-                }
-            }
-            wave_lds_sync();
         }
+        wave_lds_sync();  // leaf tables are rewritten by the next batch
     }
+
+    // ---- phase 3: strided tiles of the staged candidates
+    const int nc = nkept < a.wcap ? nkept : a.wcap;
+    const int T = (nc + WAVE - 1) / WAVE;
+    // the staging writes of all lanes before their reads by other lanes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    for(int tt = 0; tt < T; ++tt) {
+        const int c = tt + lane * T;
+        const bool has = c < nc;
+        const int m = (nc - tt + T - 1) / T;  // candidates of this tile (a prefix of its lanes)
+        int j = -1;
+        if(has) {
+            j = a.ws_j[wbase + c];
+#pragma unroll
+            for(int k = 0; k < D; ++k) c_x[w][k][lane] = s.f[F_X + k][j];
+            if constexpr(IJ) c_lim[w][lane] = a.ws_lim[wbase + c];
+        }
+        a.cand[gbase + (size_t)tt * WAVE + lane] = j;
+        wave_lds_sync();
+        // every lane tests every candidate of the tile (uniform loop,
+        // broadcast LDS reads) into a hit mask, and a mask of the hits
+        // inside its own radius (the gather entries of a split search;
+        // every hit otherwise)
+        uint64_t mine = 0, mfront = 0;
+        if(valid) {
+            uint32_t mlo = 0, mhi = 0, flo = 0, fhi = 0;
+            auto test = [&](int cc, uint32_t &hb, uint32_t &fb) {
+                double xj[D], d[D];
+#pragma unroll
+                for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][cc];
+                const double r2 = sep2<D>(box, xi, xj, d);
+                // r2 < max(h_i^2, kl^2) as two tests; all three are
+                // non-negative, so their bit patterns order as integers
+                const int64_t rb = __double_as_longlong(r2);
+                const bool f = rb < h2_bits;
+                bool hh = f;
+                if constexpr(IJ) hh = f | (rb < __double_as_longlong(c_lim[w][cc]));
+                hb = (uint32_t)hh;
+                fb = (uint32_t)f;
+            };
+            const int m1 = m < 32 ? m : 32;
+            for(int cc = 0; cc < m1; ++cc) {
+                uint32_t hb, fb;
+                test(cc, hb, fb);
+                mlo |= hb << cc;
+                flo |= fb << cc;
+            }
+            for(int cc = 32; cc < m; ++cc) {
+                uint32_t hb, fb;
+                test(cc, hb, fb);
+                mhi |= hb << (cc - 32);
+                fhi |= fb << (cc - 32);
+            }
+            mine = ((uint64_t)mhi << 32) | mlo;
+            mfront = a.split ? ((uint64_t)fhi << 32) | flo : mine;
+        }
+        const size_t mo = gbase + (size_t)tt * WAVE + lane;
+        a.mf[mo] = mfront;
+        a.mb[mo] = mine & ~mfront;
+        count += __popcll(mfront);
+        count2 += __popcll(mine & ~mfront);
+        wave_lds_sync();
+    }
+    if(lane == 0) a.gnc[grp] = nc;
 
     int over = 0, len = 0;
     if(valid) {
-        // past the capacity the first cap entries found are kept and the
-        // overflow is reported
-        a.cnt[i] = wf;
-        a.cnt2[i] = wb;
+        // past the reference's list capacity (its unchecked write,
+        // src/bhtree.cpp:257-258) the search reports an overflow
         over = count + count2 > a.cap ? 1 : 0;
         len = count + count2;
     }
@@ -417,6 +346,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         atomicAdd((unsigned long long *)&a.counters[C_SEARCH_NODES], (unsigned long long)visited);
         atomicAdd((unsigned long long *)&a.counters[C_SEARCH_CAND], (unsigned long long)tested);
         if(over) atomicAdd((unsigned long long *)&a.counters[C_OVERFLOW], (unsigned long long)over);
+        if(nkept > a.wcap) atomicAdd((unsigned long long *)&a.counters[C_TILE_OVER], 1ull);
         atomicMax((unsigned long long *)&a.counters[C_MAXLIST], (unsigned long long)len);
     }
     }  // groups
@@ -467,54 +397,135 @@ int search_guess(Ctx &c)
     return SPHGPU_OK;
 }
 
+static SearchArgs base_args(Ctx &c)
+{
+    SearchArgs a{};
+    a.cand = c.tile_cand;
+    a.mf = c.tile_mf;
+    a.mb = c.tile_mb;
+    a.gnc = c.tile_nc;
+    a.wcap = c.tile_wcap;
+    a.ws_j = c.ws_j;
+    a.ws_lim = c.ws_lim;
+    a.cap = c.cap;
+    a.counters = c.counters;
+    return a;
+}
+
 // One walk for both sets: with sml = h_guess * ratio and the cells' largest
 // sml from tree_set_kernel, the test r < max(sml_i, sml_j) finds every
-// gather candidate (r < sml_i, kept at the front with |r_ij|) and every pair
-// with r < max(h_i, h_j) for the final h, as long as no final h exceeds its
+// gather candidate (r < sml_i, in the front masks) and every pair with
+// r < max(h_i, h_j) for the final h, as long as no final h exceeds its
 // search radius (the caller checks the truncation counter).
 int search_combined(Ctx &c)
 {
-    SearchArgs a{};
-    a.list = c.list;
-    a.list_r = c.list_r;
-    a.list_rf = c.list_rf;
-    a.cnt = c.list_cnt;
-    a.cnt2 = c.list_cnt2;
+    SearchArgs a = base_args(c);
     a.split = 1;
-    a.cap = c.cap;
     a.guess = 0;
-    a.counters = c.counters;
     return search_dispatch<true>(c, a);
 }
 
 int search_gather(Ctx &c, bool initial)
 {
-    SearchArgs a{};
-    a.list = c.list;
-    a.list_r = c.list_r;
-    a.list_rf = c.list_rf;
-    a.cnt = c.list_cnt;
-    a.cnt2 = c.list_cnt2;
-    a.cap = c.cap;
+    SearchArgs a = base_args(c);
     a.guess = 1;
     a.ratio = initial ? 1.0 : (c.p.iterative_sml ? 1.2 : 1.0);  // src/pre_interaction.cpp:31-35
     a.nngb = (double)c.p.neighbor_number;
-    a.counters = c.counters;
     return search_dispatch<false>(c, a);
 }
 
 int search_symmetric(Ctx &c)
 {
-    SearchArgs a{};
-    a.list = c.list;
-    a.list_r = nullptr;
-    a.list_rf = nullptr;
-    a.cnt = c.list_cnt;
-    a.cnt2 = c.list_cnt2;
-    a.cap = c.cap;
+    SearchArgs a = base_args(c);
     a.guess = 0;
-    a.counters = c.counters;
     return search_dispatch<true>(c, a);
+}
+
+// Slabs of candidate tiles for the c.ngroups groups of the last tree, and
+// the staging buffers of the search's persistent waves.  A slab holds
+// tile_wcap candidates: the larger of eight times the per-particle capacity
+// and 48 times neighborNumber.  A group of 64 particles keeps about 19
+// times neighborNumber in 3D (1,220 at 64 neighbours in the Evrard
+// benchmark).
+int search_alloc(Ctx &c)
+{
+    const int64_t want = std::max<int64_t>({(int64_t)8 * c.cap, (int64_t)48 * c.p.neighbor_number, 1024});
+    const int wcap = (int)std::min<int64_t>((want + WAVE - 1) / WAVE * WAVE, 1 << 22);
+    const int64_t waves = (int64_t)8 * c.num_cu * SW + SW;
+    if(wcap != c.tile_wcap || waves > c.ws_waves) {
+        if(c.ws_j) (void)hipFree(c.ws_j);
+        if(c.ws_lim) (void)hipFree(c.ws_lim);
+        c.ws_j = nullptr;
+        c.ws_lim = nullptr;
+        SPHGPU_HIP(hipMalloc(&c.ws_j, (size_t)waves * wcap * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.ws_lim, (size_t)waves * wcap * sizeof(double)));
+        c.ws_waves = waves;
+    }
+    if(wcap != c.tile_wcap || c.ngroups > c.tile_groups_cap) {
+        // grow with headroom: the group count changes a little every step
+        const int64_t groups = std::max<int64_t>((int64_t)c.ngroups + c.ngroups / 4 + 64, c.tile_groups_cap);
+        if(c.tile_cand) (void)hipFree(c.tile_cand);
+        if(c.tile_mf) (void)hipFree(c.tile_mf);
+        if(c.tile_mb) (void)hipFree(c.tile_mb);
+        if(c.tile_nc) (void)hipFree(c.tile_nc);
+        c.tile_cand = nullptr;
+        c.tile_mf = c.tile_mb = nullptr;
+        c.tile_nc = nullptr;
+        const size_t slots = (size_t)groups * wcap;
+        SPHGPU_HIP(hipMalloc(&c.tile_cand, slots * sizeof(int)));
+        SPHGPU_HIP(hipMalloc(&c.tile_mf, slots * sizeof(uint64_t)));
+        SPHGPU_HIP(hipMalloc(&c.tile_mb, slots * sizeof(uint64_t)));
+        SPHGPU_HIP(hipMalloc(&c.tile_nc, groups * sizeof(int)));
+        c.tile_groups_cap = groups;
+    }
+    c.tile_wcap = wcap;
+    return SPHGPU_OK;
+}
+
+// Lists in list_at layout from the tiles of the last search: lane l of a
+// group appends its front hits from slot 0 on and its back hits from slot
+// cap - 1 down, both in staging order (candidate c = t + kT is bit k of
+// tile t), which is the order the reference's one-leaf-at-a-time walk
+// finds them in.  cnt / cnt2 receive the counts (cnt2 may be null with
+// front_only).  Entries past cap are dropped (the search reports them).
+__global__ void __launch_bounds__(BLOCK) expand_kernel(const int2 *groups, int ngroups, const int *gnc, const int *cand,
+                                                       const uint64_t *mf, const uint64_t *mb, int wcap, int cap,
+                                                       int front_only, int *list, int *cnt, int *cnt2)
+{
+    const int grp = xcd_block() * (BLOCK / WAVE) + (int)(threadIdx.x / WAVE);
+    if(grp >= ngroups) return;
+    const int lane = wave_lane();
+    const int2 gr = groups[grp];
+    const int i = gr.x + lane;
+    if(i >= gr.y) return;
+    const size_t gbase = (size_t)grp * wcap;
+    const int nc = gnc[grp];
+    const int T = (nc + WAVE - 1) / WAVE;
+    const size_t lb = list_base(i, cap);
+    int wf = 0, wb = 0;  // entries written (at most cap in total)
+    for(int k = 0; k < WAVE; ++k) {
+        for(int t = 0; t < T; ++t) {
+            if(t + k * T >= nc) break;
+            const size_t o = gbase + (size_t)t * WAVE;
+            const uint64_t bit = 1ull << k;
+            const bool room = wf + wb < cap;
+            if(mf[o + lane] & bit) {
+                if(room) list[lb + list_off(wf++)] = cand[o + k];
+            } else if(!front_only && (mb[o + lane] & bit)) {
+                if(room) list[lb + list_off(cap - 1 - wb++)] = cand[o + k];
+            }
+        }
+    }
+    cnt[i] = wf;
+    if(cnt2) cnt2[i] = wb;
+}
+
+int search_expand(Ctx &c, int *list, int *cnt, int *cnt2, bool front_only)
+{
+    hipLaunchKernelGGL(expand_kernel, dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.groups, c.ngroups, c.tile_nc,
+                       c.tile_cand, c.tile_mf, c.tile_mb, c.tile_wcap, c.cap, front_only ? 1 : 0, list, cnt, cnt2);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
 }
 
 }  // namespace sphgpu

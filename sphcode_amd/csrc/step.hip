@@ -16,17 +16,20 @@ template <int D> __global__ void __launch_bounds__(BLOCK) init_kernel(State s, i
 }
 
 // TimeStep::calculation (src/timestep.cpp:18-38), first pass: per-block
-// min of c_force sqrt(h / |a|) over |a| > 0.
-template <int D> __global__ void __launch_bounds__(BLOCK) dt_partial_kernel(State s, int n, double cf, double *partial)
+// min of c_force sqrt(h / |a|) over |a| > 0, grid-stride over the particles
+// (at most 1024 blocks, one partial each).
+template <int D>
+__global__ void __launch_bounds__(BLOCK) dt_partial_kernel(const double *sml, const double *ax, const double *ay,
+                                                           const double *az, int n, double cf, double *partial)
 {
     double m = 1.7976931348623157e308;
     for(int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        double a2 = s.f[F_A][i] * s.f[F_A][i];
-#pragma unroll
-        for(int k = 1; k < D; ++k) a2 += s.f[F_A + k][i] * s.f[F_A + k][i];
+        double a2 = ax[i] * ax[i];
+        if constexpr(D > 1) a2 += ay[i] * ay[i];
+        if constexpr(D > 2) a2 += az[i] * az[i];
         const double a = sqrt(a2);
         if(a > 0.0) {
-            const double d = cf * sqrt(s.f[F_SML][i] / a);
+            const double d = cf * sqrt(sml[i] / a);
             m = d < m ? d : m;
         }
     }
@@ -41,7 +44,7 @@ template <int D> __global__ void __launch_bounds__(BLOCK) dt_partial_kernel(Stat
 }
 
 // Second pass: dt = min(c_sound * min(h / v_sig), min dt_force).
-__global__ void dt_final_kernel(const double *partial, int nb, double csound, double *scal)
+__global__ void dt_final_kernel(const double *partial, int nb, double csound, const double *h_per_v_sig, double *dt)
 {
     __shared__ double sh[BLOCK / WAVE];
     double m = 1.7976931348623157e308;
@@ -52,9 +55,23 @@ __global__ void dt_final_kernel(const double *partial, int nb, double csound, do
     if(threadIdx.x == 0) {
         for(int w = 1; w < (int)(blockDim.x / WAVE); ++w) m = sh[w] < m ? sh[w] : m;
         m = sh[0] < m ? sh[0] : m;
-        const double ds = csound * scal[S_HVS];
-        scal[S_DT] = ds < m ? ds : m;
+        const double ds = csound * *h_per_v_sig;
+        *dt = ds < m ? ds : m;
     }
+}
+
+// The two passes over one state (grid of at most DT_MAX_BLOCKS blocks).
+constexpr int DT_MAX_BLOCKS = 1024;
+static void launch_dt(int D, int n, const double *sml, const double *const acc[3], const double *hvs, double cf,
+                      double cs, double *partial, double *dt, hipStream_t st)
+{
+    const int nb = std::min(grid_for(n), DT_MAX_BLOCKS);
+    switch(D) {
+    case 1: hipLaunchKernelGGL(dt_partial_kernel<1>, dim3(nb), dim3(BLOCK), 0, st, sml, acc[0], acc[1], acc[2], n, cf, partial); break;
+    case 2: hipLaunchKernelGGL(dt_partial_kernel<2>, dim3(nb), dim3(BLOCK), 0, st, sml, acc[0], acc[1], acc[2], n, cf, partial); break;
+    default: hipLaunchKernelGGL(dt_partial_kernel<3>, dim3(nb), dim3(BLOCK), 0, st, sml, acc[0], acc[1], acc[2], n, cf, partial); break;
+    }
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(BLOCK), 0, st, partial, nb, cs, hvs, dt);
 }
 
 __global__ void min_final_kernel(const double *partial, int nb, double *scal, int slot)
@@ -223,9 +240,9 @@ int reduce_min_partials(Ctx &c, int nblocks, int slot)
 
 int step_timestep(Ctx &c)
 {
-    const int nb = std::min(grid_for(c.n), 1024);
-    SPHGPU_DISPATCH_D(c, dt_partial_kernel, nb, c.s, (int)c.n, c.p.cfl_force, c.partial);
-    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.partial, nb, c.p.cfl_sound, c.scal);
+    const double *acc[3] = {c.s.f[F_A], c.D > 1 ? c.s.f[F_A + 1] : nullptr, c.D > 2 ? c.s.f[F_A + 2] : nullptr};
+    launch_dt(c.D, (int)c.n, c.s.f[F_SML], acc, c.scal + S_HVS, c.p.cfl_force, c.p.cfl_sound, c.partial,
+              c.scal + S_DT, c.stream);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }

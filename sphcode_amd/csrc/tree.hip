@@ -481,8 +481,9 @@ int tree_alloc(Ctx &c)
         SPHGPU_HIP(hipMalloc(&c.leafcnt, waves * sizeof(int)));
     }
     if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
-    if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, 2 * NLEVEL * sizeof(int), hipHostMallocDefault));
-    for(int k = 0; k < 2 * NLEVEL; ++k) c.lvl_host[k] = 0;
+    // per-level counts and offsets, then the number of search groups
+    if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, (2 * NLEVEL + 1) * sizeof(int), hipHostMallocDefault));
+    for(int k = 0; k < 2 * NLEVEL + 1; ++k) c.lvl_host[k] = 0;
     c.group_cap = c.node_cap + n / 64 + 1;
     SPHGPU_HIP(hipMalloc(&c.groups, c.group_cap * sizeof(int2)));
     SPHGPU_HIP(hipMalloc(&c.num_groups, sizeof(int)));
@@ -543,18 +544,23 @@ template <int D> static int tree_build_d(Ctx &c)
     hipLaunchKernelGGL(level_count_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl);
     hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.lvl);
     hipLaunchKernelGGL(level_scatter_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl, c.lvl_nodes);
+    // search groups (they need the node counts and parent links only), so
+    // that their number comes to the host with the level counts
+    hipLaunchKernelGGL(reset_groups_kernel, dim3(1), dim3(1), 0, c.stream, c.num_groups);
+    hipLaunchKernelGGL(groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.groups, c.num_groups, c.group_cap);
     SPHGPU_HIP(hipMemcpyAsync(c.lvl_host, c.lvl, 2 * NLEVEL * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipMemcpyAsync(c.lvl_host + 2 * NLEVEL, c.num_groups, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    c.ngroups = c.lvl_host[2 * NLEVEL];
+    if(c.ngroups < 1 || c.ngroups > c.group_cap) return set_error(SPHGPU_E_TREE, "search groups out of range");
     for(int L = NLEVEL - 1; L >= 0; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
         hipLaunchKernelGGL(moments_level_kernel<D>, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
                            c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
     }
-    hipLaunchKernelGGL(reset_groups_kernel, dim3(1), dim3(1), 0, c.stream, c.num_groups);
-    hipLaunchKernelGGL(groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.groups, c.num_groups, c.group_cap);
     SPHGPU_HIP(hipGetLastError());
-    return SPHGPU_OK;
+    return search_alloc(c);
 }
 
 int tree_build(Ctx &c)

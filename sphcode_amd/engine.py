@@ -136,13 +136,18 @@ class Engine:
     def set_record_sets(self, on: bool) -> None:
         self._call("set_record_sets", C.c_int(1 if on else 0))
 
-    def neighbor_sets(self, kind: int, cap: int | None = None) -> list[np.ndarray]:
+    def neighbor_sets_packed(self, kind: int, cap: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """(offsets, members): the set of particle i is members[offsets[i]:offsets[i+1]]."""
         cap = cap or self.n * max(64, int(self.params.neighbor_number) * 8)
         counts = np.zeros(self.n, np.int32)
         lists = np.zeros(cap, np.int32)
         self._call("neighbor_sets", C.c_int(kind), counts.ctypes.data_as(C.POINTER(C.c_int32)),
                    lists.ctypes.data_as(C.POINTER(C.c_int32)), C.c_int64(cap))
         offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+        return offs, lists
+
+    def neighbor_sets(self, kind: int, cap: int | None = None) -> list[np.ndarray]:
+        offs, lists = self.neighbor_sets_packed(kind, cap)
         return [lists[offs[i]:offs[i + 1]].copy() for i in range(self.n)]
 
     def tree_info(self, cap: int | None = None) -> dict:

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "sph/module.hpp"
+#include "sph/output.hpp"
 
 namespace sph {
 
@@ -33,8 +34,10 @@ public:
     void restart(const std::vector<SPHParticle> &particles, double time, double dt, double h_per_v_sig);
     // Solver::integrate + update_time (src/solver.cpp:417-429)
     void integrate();
-    // integrate() until endTime; returns the number of steps
-    int run();
+    // integrate() until endTime, writing particles and energies through out
+    // when given (the loop of src/solver.cpp:300-351); returns the number of
+    // steps
+    int run(Output *out = nullptr);
     std::shared_ptr<Simulation> simulation() const { return m_sim; }
 
 private:

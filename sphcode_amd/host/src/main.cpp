@@ -6,8 +6,12 @@
 //
 //   sphgpu_run PARAMS.json --print-params
 //   sphgpu_run PARAMS.json --particles IN.snap [--restart] [--steps K] [--output OUT.snap] [--energy]
+//              [--output-dir DIR]
 //
-// Without --steps the run goes to endTime (the loop of src/solver.cpp:318).
+// Without --steps the run goes to endTime (the loop of src/solver.cpp:318);
+// with --output-dir it writes, as Solver::run does, DIR/energy.dat at the
+// energyTime cadence and the particles at the outputTime cadence
+// (full-precision snapshots, sph/output.hpp).
 // With --restart IN.snap is a state written by --output after a full step:
 // the run continues from its time, dt and min h/v_sig instead of
 // initialising (the reference has no restart).
@@ -15,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "sph/exception.hpp"
@@ -62,7 +67,7 @@ void print_energy(const sph::Simulation &sim, int loop) {
 int usage() {
     std::fprintf(stderr, "usage: sphgpu_run PARAMS.json --print-params\n"
                          "       sphgpu_run PARAMS.json --particles IN.snap [--restart] [--steps K] "
-                         "[--output OUT.snap] [--energy]\n");
+                         "[--output OUT.snap] [--energy] [--output-dir DIR]\n");
     return 2;
 }
 
@@ -71,7 +76,7 @@ int usage() {
 int main(int argc, char **argv) {
     if (argc < 3) return usage();
     const std::string param_file = argv[1];
-    std::string particles, output;
+    std::string particles, output, output_dir;
     int steps = -1;
     bool show = false, energy = false, restart = false;
     for (int a = 2; a < argc; ++a) {
@@ -80,6 +85,7 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(argv[a], "--restart")) restart = true;
         else if (!std::strcmp(argv[a], "--particles") && a + 1 < argc) particles = argv[++a];
         else if (!std::strcmp(argv[a], "--output") && a + 1 < argc) output = argv[++a];
+        else if (!std::strcmp(argv[a], "--output-dir") && a + 1 < argc) output_dir = argv[++a];
         else if (!std::strcmp(argv[a], "--steps") && a + 1 < argc) steps = std::atoi(argv[++a]);
         else return usage();
     }
@@ -104,7 +110,9 @@ int main(int argc, char **argv) {
         if (energy) print_energy(*sim, 0);
         int loop = 0;
         if (steps < 0) {
-            loop = solver.run();
+            std::unique_ptr<sph::Output> out;
+            if (!output_dir.empty()) out.reset(new sph::Output(output_dir));
+            loop = solver.run(out.get());
         } else {
             for (; loop < steps; ++loop) {
                 solver.integrate();
@@ -113,13 +121,12 @@ int main(int argc, char **argv) {
         }
         std::printf("steps=%d time=%.17g\n", loop, sim->get_time());
         if (!output.empty()) {
-            sim->sync_from_device();
             sph::Snapshot out;
             out.dim = param->dim;
             out.time = sim->get_time();
             out.dt = sim->get_dt();
             out.h_per_v_sig = sim->get_h_per_v_sig();
-            out.particles = sim->get_particles();
+            out.particles = sim->particles_view();
             sph::write_snapshot(output, out);
         }
     } catch (const sph::SPHException &e) {

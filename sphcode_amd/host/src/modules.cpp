@@ -37,6 +37,7 @@ void PreInteraction::calculation(std::shared_ptr<Simulation> sim) {
     need(sim);
     check_type(sim->params().type, expected(), "PreInteraction");
     check_status(sphgpu_pre_interaction(sim->device()), "sphgpu_pre_interaction");
+    sim->note_pre_interaction();
 }
 
 void FluidForce::initialize(std::shared_ptr<SPHParameters> param) {

@@ -22,7 +22,7 @@ template <class T> T get(std::ifstream &f, const std::string &path) {
 }
 
 // member accessors in file order
-using VecField = double (SPHParticle::*)[3];
+using VecField = vec_t SPHParticle::*;
 using ScalarField = double SPHParticle::*;
 using IntField = int SPHParticle::*;
 const VecField kVec[] = {&SPHParticle::pos, &SPHParticle::vel, &SPHParticle::vel_p, &SPHParticle::acc};

@@ -68,15 +68,31 @@ void Solver::integrate() {
     m_sim->update_time();
 }
 
-// the loop of Solver::run (src/solver.cpp:318-346) without output
-int Solver::run() {
+// the loop of Solver::run (src/solver.cpp:300-351): integrate until
+// endTime; with an Output, the particles and the energies at the start and
+// after every step that passes the next outputTime / energyTime mark
+int Solver::run(Output *out) {
     if (!m_sim) throw SPHException("Solver::run before initialize");
+    double t_out = m_param->time.output;
+    double t_ene = m_param->time.energy;
+    if (out) {
+        out->output_particle(*m_sim);
+        out->output_energy(*m_sim);
+    }
     int loop = 0;
     double t = m_sim->get_time();
     while (t < m_param->time.end) {
         integrate();
         ++loop;
         t = m_sim->get_time();
+        if (out && t > t_out) {
+            out->output_particle(*m_sim);
+            t_out += m_param->time.output;
+        }
+        if (out && t > t_ene) {
+            out->output_energy(*m_sim);
+            t_ene += m_param->time.energy;
+        }
     }
     return loop;
 }

@@ -66,6 +66,22 @@ def evrard_ssph():
     return p, f
 
 
+def evrard_ac():
+    # artificial conductivity with gravity on: its signal speed is then
+    # |v_ij . r_ij| / r (src/fluid_force.cpp:111-112); an infall velocity
+    # field with noise makes it nonzero from the first force evaluation
+    import numpy as np
+    p, f = ics.evrard(4096, leaf=32)
+    p.ac_is_valid = 1
+    p.ac_alpha = 1.0
+    x = f["pos"]
+    r = np.sqrt((x * x).sum(1))[:, None]
+    rng = np.random.default_rng(5)
+    f = dict(f)
+    f["vel"] = -0.2 * x / np.maximum(r, 1e-3) + rng.normal(0.0, 0.05, x.shape)
+    return p, f
+
+
 def rollout_ssph():
     return ics.rollout(n_side=12)
 
@@ -96,6 +112,7 @@ CASES = {
     "khi_ssph": khi_ssph,
     "evrard_disph": evrard_disph,
     "evrard_ssph": evrard_ssph,
+    "evrard_ac": evrard_ac,
     "rollout_ssph": rollout_ssph,
     "rollout_dense": rollout_dense,
     "sedov_gsph": sedov_gsph,

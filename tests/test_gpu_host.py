@@ -145,3 +145,34 @@ def test_restart_rejects_fresh_snapshot(gpu, tmp_path):
     r = subprocess.run([str(RUN), str(pj), "--particles", str(sin), "--steps", "1", "--restart"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "--restart" in r.stderr, r.stderr
+
+
+@pytest.mark.parametrize("case", ["sod_gsph2", "evrard_disph", "khi_disph"])
+def test_reference_shaped_solver_runs(gpu, tmp_path, case):
+    """tests/host_solver_mirror.cpp drives the GPU modules the way the
+    reference's Solver does (src/solver.cpp:353-474): predict and correct
+    as host loops over Simulation::get_particles(), the tree through
+    get_tree()->resize / make and make_tree.  Each host change is uploaded
+    before the next stage (continuing the run, as a restart does), so three
+    such steps reach sphgpu_run's three fused steps to round-off (particles
+    sharing a Morton key may be summed in another order after an upload)."""
+    from test_host import build_mirror
+    exe = build_mirror(tmp_path)
+    p, f = CASES[case]()
+    p.time_end = 1.0
+    pj, sin = tmp_path / "p.json", tmp_path / "in.snap"
+    snapshot.write_params(pj, p)
+    snapshot.write(sin, f, int(p.dim))
+    r = subprocess.run([str(exe), str(pj), str(sin), "3", str(tmp_path / "mirror.snap")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got, _, t, dt, hv = snapshot.read(tmp_path / "mirror.snap", with_hvs=True)
+    want, _, t3, dt3, hv3 = _cli(tmp_path, pj, sin, tmp_path / "three.snap", 3, False)
+    np.testing.assert_allclose(t, t3, rtol=1e-12)
+    np.testing.assert_allclose(dt, dt3, rtol=1e-10)
+    np.testing.assert_allclose(hv, hv3, rtol=1e-10)
+    for k in ("pos", "vel", "ene", "sml", "dens", "pres", "alpha", "acc", "dene"):
+        scale = np.abs(want[k]).max() + 1e-300
+        err = np.abs(got[k] - want[k]).max() / scale
+        print(f"[mirror] {case} {k}: max rel err {err:.2e}", flush=True)
+        assert err < 1e-10, f"{case}: {k} max rel err {err:.3e}"

@@ -149,3 +149,78 @@ def test_snapshot_round_trip(tmp_path):
     for k, v in fields.items():
         np.testing.assert_array_equal(got[k], v)
     assert f.stat().st_size == 48 + n * (8 * (4 * dim + 12) + 8)
+
+
+MIRROR_SRC = ROOT / "tests" / "host_solver_mirror.cpp"
+
+
+def build_mirror(out_dir: pathlib.Path) -> pathlib.Path:
+    """Compile and link tests/host_solver_mirror.cpp (a solver written in the
+    shape of the reference's Solver, src/solver.cpp:353-474) against
+    libsphhost and libsphgpu."""
+    exe = out_dir / "host_solver_mirror"
+    lib = HOST_LIB.parent
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter",
+           "-I", str(ROOT / "include"), "-I", str(ROOT / "sphcode_amd" / "host" / "include"),
+           str(MIRROR_SRC), "-o", str(exe), "-L", str(lib), "-lsphhost", "-lsphgpu", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_reference_shaped_solver_compiles_and_links(tmp_path):
+    """A driver using the reference's Simulation surface -- get_particles()
+    with vec_t positions, set_particle_num, add_vector_array, get_tree()
+    with resize and make, get_periodic()->apply, get_dt, make_tree -- and
+    the Module interface compiles and links against the host library."""
+    exe = build_mirror(tmp_path)
+    out = subprocess.run(["nm", "-C", "--undefined-only", str(exe)], capture_output=True, text=True, check=True).stdout
+    for sym in ("sph::Simulation::get_particles()", "sph::BHTree::make(", "sph::Simulation::add_vector_array(",
+                "sph::Simulation::make_tree()"):
+        assert sym in out, sym
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_host_kernels_match_oracle():
+    """The host KernelFunction implementations (for code working on
+    get_particles()) equal the oracle's kernels."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _oracle import kernel_eval
+    src = r'''
+#include <cstdio>
+#include "sph/kernel_function.hpp"
+#include "sph/periodic.hpp"
+int main() {
+    for (int dim = 1; dim <= 3; ++dim)
+        for (int kern = 0; kern < 2; ++kern) {
+            if (kern == 1 && dim == 1) continue;
+            sph::SPHParameters p;
+            p.dim = dim;
+            p.kernel = kern ? sph::KernelType::WENDLAND : sph::KernelType::CUBIC_SPLINE;
+            auto k = sph::make_kernel(p);
+            for (double r : {0.0, 0.1, 0.37, 0.5, 0.81, 1.2}) {
+                const double h = 1.3;
+                sph::vec_t rij(r, 0.0, 0.0);
+                std::printf("%d %d %.17g %.17g %.17g %.17g\n", dim, kern, r, k->w(r, h), k->dw(rij, r, h)[0], k->dhw(r, h));
+            }
+        }
+    return 0;
+}
+'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        f = pathlib.Path(d) / "k.cpp"
+        f.write_text(src)
+        exe = pathlib.Path(d) / "k"
+        subprocess.run(["g++", "-std=c++17", "-O2", "-I", str(ROOT / "include"), "-I",
+                        str(ROOT / "sphcode_amd" / "host" / "include"), str(f), "-o", str(exe)], check=True)
+        lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for ln in filter(None, lines):
+        dim, kern, r, w, dwx, dh = ln.split()
+        dim, kern, r, w, dwx, dh = int(dim), int(kern), float(r), float(w), float(dwx), float(dh)
+        ow, oc, odh = kernel_eval(dim, kern, r, 1.3)
+        assert w == pytest.approx(ow, rel=1e-13, abs=1e-15)
+        assert dwx == pytest.approx(oc * r, rel=1e-13, abs=1e-15)
+        assert dh == pytest.approx(odh, rel=1e-12, abs=1e-14)
