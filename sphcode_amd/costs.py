@@ -49,3 +49,10 @@ def pair_flops(stats: dict, n: int, steps: int, sph_type: str = "disph") -> tupl
     pre = NEWTON_FLOP_PAIR * gp * iters + (DENSITY_FLOP_PAIR + BALSARA_FLOP_PAIR) * gp
     per_force = {"disph": DISPH_FORCE_FLOP_PAIR, "ssph": SSPH_FORCE_FLOP_PAIR, "gsph2": GSPH2_FORCE_FLOP_PAIR}[sph_type]
     return pre, per_force * stats["force_pairs"] / steps
+
+
+def gravity_flops(stats: dict) -> float:
+    """Algorithmic FLOPs of the gravity stage from its interaction counters."""
+    tests = stats["gravity_pn"] + stats["gravity_opened"]
+    return (GRAV_FLOP_PP * stats["gravity_pp"] + GRAV_FLOP_PN * stats["gravity_pn"]
+            + GRAV_FLOP_OPEN * tests)
