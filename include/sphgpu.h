@@ -292,6 +292,26 @@ int sphgpu_rollout(void *ctx, int steps, double dt);
 int sphgpu_rollout_adjoint(void *ctx, const double *dl_dx, const double *dl_dv, const double *dl_du,
                            double *dx0, double *dv0, double *du0);
 
+/*
+ * TimeStep::calculation over caller-owned device arrays, allocating nothing.
+ * No counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  It
+ * runs the kernels of sphgpu_timestep on `stream` (NULL: the null stream)
+ * and does not synchronise:
+ *   *dt = min(cfl_sound * *h_per_v_sig, cfl_force * min_i sqrt(h_i / |a_i|))
+ * over the particles with |a_i| > 0 (src/timestep.cpp:18-38).  sml and the
+ * acceleration components (acc_y for dim >= 2, acc_z for dim 3) are n
+ * doubles each, in any particle order; h_per_v_sig is one device double;
+ * dt receives one device double.  The block partials live in a 1024-double
+ * array that the library holds per GPU for the life of the process, so calls
+ * that overlap on one GPU (different streams or host threads) race.
+ * n must be in [1, SPHGPU_NOALLOC_MAX_N]; SPHGPU_E_INVALID otherwise.
+ */
+#define SPHGPU_NOALLOC_MAX_N ((int64_t)1 << 30)
+int sphgpu_timestep_noalloc(int dim, int64_t n, const double *sml, const double *acc_x, const double *acc_y,
+                            const double *acc_z, const double *h_per_v_sig, double cfl_sound, double cfl_force,
+                            double *dt, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

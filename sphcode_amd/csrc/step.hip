@@ -1,6 +1,8 @@
 // Streaming stages and global reductions: per-particle initialisation,
 // TimeStep, predict/correct kick-drift and the energy sums.  Reductions are
 // two-pass (block partials, then one block), in FP64, deterministic.
+#include <mutex>
+
 #include "ctx.hpp"
 
 namespace sphgpu {
@@ -283,3 +285,57 @@ int step_energy(Ctx &c, double *out)
 }
 
 }  // namespace sphgpu
+
+// ------------------------------------------------------------ no allocation
+// sphgpu_timestep_noalloc keeps its block partials in this file-scope array
+// (one copy per GPU, held until the process ends).  Host code never passes
+// the array's name to a kernel: its device address is fetched once per
+// device with hipGetSymbolAddress and kept in scratch_dev.
+namespace sphgpu {
+
+static __device__ __align__(256) double scratch[DT_MAX_BLOCKS];
+
+constexpr int kMaxDevices = 64;
+static double *scratch_dev[kMaxDevices] = {};
+static std::mutex scratch_mu;
+
+// Device address of this GPU's scratch array (the current device's), looked
+// up on the first call on that device.
+static int scratch_for_current_device(double **out)
+{
+    int dev = 0;
+    SPHGPU_HIP(hipGetDevice(&dev));
+    if(dev < 0 || dev >= kMaxDevices) return set_error(SPHGPU_E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if(!scratch_dev[dev]) {
+        double *p = nullptr;
+        SPHGPU_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        if(!p) return set_error(SPHGPU_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        scratch_dev[dev] = p;
+    }
+    *out = scratch_dev[dev];
+    return SPHGPU_OK;
+}
+
+}  // namespace sphgpu
+
+extern "C" int sphgpu_timestep_noalloc(int dim, int64_t n, const double *sml, const double *acc_x, const double *acc_y,
+                                       const double *acc_z, const double *h_per_v_sig, double cfl_sound,
+                                       double cfl_force, double *dt, void *stream)
+{
+    using namespace sphgpu;
+    if(dim < 1 || dim > 3) return set_error(SPHGPU_E_INVALID, "sphgpu_timestep_noalloc: dim must be 1, 2 or 3");
+    if(n < 1 || n > SPHGPU_NOALLOC_MAX_N)
+        return set_error(SPHGPU_E_INVALID, "sphgpu_timestep_noalloc: n must be in [1, " +
+                                               std::to_string((long long)SPHGPU_NOALLOC_MAX_N) + "], got " +
+                                               std::to_string((long long)n));
+    if(!sml || !acc_x || (dim > 1 && !acc_y) || (dim > 2 && !acc_z) || !h_per_v_sig || !dt)
+        return set_error(SPHGPU_E_INVALID, "sphgpu_timestep_noalloc: null device pointer");
+    double *partial = nullptr;
+    int rc = scratch_for_current_device(&partial);
+    if(rc) return rc;
+    const double *acc[3] = {acc_x, dim > 1 ? acc_y : nullptr, dim > 2 ? acc_z : nullptr};
+    launch_dt(dim, (int)n, sml, acc, h_per_v_sig, cfl_force, cfl_sound, partial, dt, (hipStream_t)stream);
+    SPHGPU_HIP(hipGetLastError());
+    return SPHGPU_OK;
+}
