@@ -26,6 +26,11 @@ BYTES_PER_PARTICLE = {
     "fluid_force": 152, "gravity": 96, "correct": 104,
 }
 
+# MI355X published peaks
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
 # Neighbour search: each particle's FP64 position and search radius read once
 # (32 B), and its neighbour set written once as 4-byte indices (one per
 # symmetric pair, the set the force pass consumes)
