@@ -246,8 +246,8 @@ struct Ctx {
     int *tile_nc = nullptr;
     int tile_wcap = 0;             // slots per group (multiple of 64)
     int64_t tile_groups_cap = 0;   // groups the slabs are allocated for
-    int *ws_j = nullptr;           // search waves' staging of kept candidates
-    double *ws_lim = nullptr;
+    int *ws_j = nullptr;           // search waves' staging of kept candidates:
+    double4 *ws_p = nullptr;       // index, and position with radius squared
     int64_t ws_waves = 0;          // search waves the staging is allocated for
     int ngroups = 0;               // groups of the last tree (host copy)
 

@@ -200,7 +200,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
     const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
-    const int pairs = wave_sum(wc.pairs), accepted = wave_sum(wc.accepted), opened = wave_sum(wc.opened);
+    const int pairs = wc.pairs, accepted = wc.accepted, opened = wc.opened;
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll

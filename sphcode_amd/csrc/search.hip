@@ -42,7 +42,7 @@ struct SearchArgs {
     int *gnc;            // candidates per group
     int wcap;            // slots per group slab and per staging buffer
     int *ws_j;           // staging buffers, one per persistent wave
-    double *ws_lim;      // IJ: the staged candidate's radius squared
+    double4 *ws_p;       // the staged candidate's position and (IJ) radius squared
     int split;           // symmetric search that also marks the gather set:
                          // r < sml_i in the front mask, the rest in the back
     int cap;             // per-particle capacity (the reference's list size)
@@ -211,15 +211,15 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         if(lane == 0) l_pre[w][nleaf] = ncand;
         wave_lds_sync();
 
-        // ---- phase 2: stage the collected leaves' particles, 64 at a time
-        int lc = 0;  // last leaf starting at or before tb
-        for(int tb = 0; tb < ncand; tb += WAVE) {
-            // stage: each lane fetches one candidate and keeps it only if it
-            // lies within the group's largest radius of the group box (the
-            // box contains every x_i, so each lane's own test implies this)
+        // ---- phase 2: stage the collected leaves' particles, 64 at a time.
+        // The next tile's candidates are located and their positions loaded
+        // before the current tile is tested, so the loads overlap the work.
+        int lc = 0;  // last leaf starting at or before the located tile
+        auto locate = [&](int tb, int &lo_, int &j_, double xj_[D], double &lim_, bool &has_) {
+            // each lane's candidate f = tb + lane; its leaf is the last leaf
+            // starting at or before f, found among the few leaves that start
+            // inside the tile
             const int f = tb + lane;
-            // the candidate's leaf: the last leaf starting at or before f,
-            // found among the few leaves that start inside the tile
             int lo = lc, nx = lc;
             for(int L = lc + 1; L < nleaf; ++L) {
                 const int st = uniform(l_pre[w][L]);
@@ -228,21 +228,42 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
                 lo = f >= st ? L : lo;
             }
             lc = nx;
+            has_ = f < ncand;
+            lo_ = lo;
+            j_ = has_ ? l_begin[w][lo] + (f - l_pre[w][lo]) : 0;
+            if(has_) {
+#pragma unroll
+                for(int k = 0; k < D; ++k) xj_[k] = s.f[F_X + k][j_];
+                // split mode: r < max(sml_i, sml_j) (each radius bounds that
+                // particle's final h unless truncated); otherwise the
+                // reference's r < max(h_i, leaf's largest h).  As squares:
+                // max(a, b)^2 == max(a^2, b^2) for a, b >= 0.
+                if constexpr(IJ) lim_ = a.split ? s.f[F_SML][j_] : l_ks[w][lo];
+            }
+        };
+        int c_lo = 0, c_jj = 0;
+        double c_xj[D], c_lm = 0.0;
+        bool c_has = false;
+        if(ncand > 0) locate(0, c_lo, c_jj, c_xj, c_lm, c_has);
+        for(int tb = 0; tb < ncand; tb += WAVE) {
+            int n_lo = 0, n_jj = 0;
+            double n_xj[D], n_lm = 0.0;
+            bool n_has = false;
+            if(tb + WAVE < ncand) locate(tb + WAVE, n_lo, n_jj, n_xj, n_lm, n_has);
+            // keep a candidate only if it lies within the group's largest
+            // radius of the group box (the box contains every x_i, so each
+            // lane's own test implies this)
             bool keep = false;
-            int j = 0;
-            double xj[D], ksj = 0.0;
-            if(f < ncand) {
-                j = l_begin[w][lo] + (f - l_pre[w][lo]);
+            if(c_has) {
                 double d2b = 0.0;
 #pragma unroll
                 for(int k = 0; k < D; ++k) {
-                    xj[k] = s.f[F_X + k][j];
-                    const double ex = fabs(min_image1(box, k, xj[k] - gc[k])) - ge[k];
+                    const double ex = fabs(min_image1(box, k, c_xj[k] - gc[k])) - ge[k];
                     d2b += ex > 0.0 ? ex * ex : 0.0;
                 }
                 double Hc = hmax;
                 if constexpr(IJ) {
-                    ksj = l_ks[w][lo];
+                    const double ksj = l_ks[w][c_lo];
                     Hc = ksj > Hc ? ksj : Hc;
                 }
                 keep = d2b <= Hc * Hc * (1.0 + 1e-9);
@@ -251,41 +272,57 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             if(keep) {
                 const int at = nkept + __popcll(kept & ((1ull << lane) - 1ull));
                 if(at < a.wcap) {
-                    a.ws_j[wbase + at] = j;
-                    if constexpr(IJ) {
-                        // split mode: r < max(sml_i, sml_j) (each radius bounds
-                        // that particle's final h unless truncated); otherwise
-                        // the reference's r < max(h_i, leaf's largest h).  As
-                        // squares: max(a, b)^2 == max(a^2, b^2) for a, b >= 0.
-                        const double kl = a.split ? s.f[F_SML][j] : ksj;
-                        a.ws_lim[wbase + at] = kl * kl;
-                    }
+                    a.ws_j[wbase + at] = c_jj;
+                    double4 p;
+                    p.x = c_xj[0];
+                    p.y = D > 1 ? c_xj[1] : 0.0;
+                    p.z = D > 2 ? c_xj[2] : 0.0;
+                    p.w = IJ ? c_lm * c_lm : 0.0;
+                    a.ws_p[wbase + at] = p;
                 }
             }
             const int m = uniform(__popcll(kept));
             nkept += m;
             tested += m;
+            c_lo = n_lo;
+            c_jj = n_jj;
+#pragma unroll
+            for(int k = 0; k < D; ++k) c_xj[k] = n_xj[k];
+            c_lm = n_lm;
+            c_has = n_has;
         }
         wave_lds_sync();  // leaf tables are rewritten by the next batch
     }
 
-    // ---- phase 3: strided tiles of the staged candidates
+    // ---- phase 3: strided tiles of the staged candidates, the next tile's
+    // staged records loaded while the current one is tested
     const int nc = nkept < a.wcap ? nkept : a.wcap;
     const int T = (nc + WAVE - 1) / WAVE;
     // the staging writes of all lanes before their reads by other lanes
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    int p_j = -1;
+    double4 p_p = make_double4(0.0, 0.0, 0.0, 0.0);
+    if(T > 0 && lane * T < nc) {
+        p_j = a.ws_j[wbase + lane * T];
+        p_p = a.ws_p[wbase + lane * T];
+    }
     for(int tt = 0; tt < T; ++tt) {
-        const int c = tt + lane * T;
-        const bool has = c < nc;
         const int m = (nc - tt + T - 1) / T;  // candidates of this tile (a prefix of its lanes)
-        int j = -1;
-        if(has) {
-            j = a.ws_j[wbase + c];
-#pragma unroll
-            for(int k = 0; k < D; ++k) c_x[w][k][lane] = s.f[F_X + k][j];
-            if constexpr(IJ) c_lim[w][lane] = a.ws_lim[wbase + c];
+        const int j = p_j;
+        if(j >= 0) {
+            c_x[w][0][lane] = p_p.x;
+            if constexpr(D > 1) c_x[w][1][lane] = p_p.y;
+            if constexpr(D > 2) c_x[w][2][lane] = p_p.z;
+            if constexpr(IJ) c_lim[w][lane] = p_p.w;
         }
         a.cand[gbase + (size_t)tt * WAVE + lane] = j;
+        // next tile's records
+        const int cn = tt + 1 + lane * T;
+        p_j = -1;
+        if(tt + 1 < T && cn < nc) {
+            p_j = a.ws_j[wbase + cn];
+            p_p = a.ws_p[wbase + cn];
+        }
         wave_lds_sync();
         // every lane tests every candidate of the tile (uniform loop,
         // broadcast LDS reads) into a hit mask, and a mask of the hits
@@ -406,7 +443,7 @@ static SearchArgs base_args(Ctx &c)
     a.gnc = c.tile_nc;
     a.wcap = c.tile_wcap;
     a.ws_j = c.ws_j;
-    a.ws_lim = c.ws_lim;
+    a.ws_p = c.ws_p;
     a.cap = c.cap;
     a.counters = c.counters;
     return a;
@@ -454,11 +491,11 @@ int search_alloc(Ctx &c)
     const int64_t waves = (int64_t)8 * c.num_cu * SW + SW;
     if(wcap != c.tile_wcap || waves > c.ws_waves) {
         if(c.ws_j) (void)hipFree(c.ws_j);
-        if(c.ws_lim) (void)hipFree(c.ws_lim);
+        if(c.ws_p) (void)hipFree(c.ws_p);
         c.ws_j = nullptr;
-        c.ws_lim = nullptr;
+        c.ws_p = nullptr;
         SPHGPU_HIP(hipMalloc(&c.ws_j, (size_t)waves * wcap * sizeof(int)));
-        SPHGPU_HIP(hipMalloc(&c.ws_lim, (size_t)waves * wcap * sizeof(double)));
+        SPHGPU_HIP(hipMalloc(&c.ws_p, (size_t)waves * wcap * sizeof(double4)));
         c.ws_waves = waves;
     }
     if(wcap != c.tile_wcap || c.ngroups > c.tile_groups_cap) {

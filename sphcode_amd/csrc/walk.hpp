@@ -33,13 +33,14 @@ constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 //                                        for the lanes with `open`
 // The walk state (cursor, masks, parked stack) is wave-uniform and kept in
 // scalar registers; each node is one 64-byte record (GravNode) read with
-// one scalar load.  WalkCount returns the wave's node visits and the
-// particles of the leaves that some lane opened (wave-uniform), and the
-// lane's own opened nodes, accepted nodes and particle interactions (per
-// lane, counted in vector registers to keep the scalar unit free).
+// one scalar load.  WalkCount returns the wave's node visits, the
+// particles of the leaves that some lane opened, and the lanes' opened
+// nodes, accepted nodes and particle interactions, all wave totals counted
+// from the lane masks with scalar bit counts (no vector instructions per
+// visit).
 struct WalkCount {
-    int visits = 0, slots = 0;          // per wave
-    int opened = 0, accepted = 0, pairs = 0;  // per lane
+    int visits = 0, slots = 0;
+    int opened = 0, accepted = 0, pairs = 0;
 };
 
 template <class V>
@@ -81,12 +82,13 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
         const int skip = g.skip;
         const bool open = v.node(g, me);
         const uint64_t om = uniform64(__ballot(open));
-        wc.opened += open;
-        wc.accepted += me && !open;
+        const int nopen = __popcll(om);
+        wc.opened += nopen;
+        wc.accepted += __popcll(active) - nopen;
         if(g.leaf) {
             if(om) {
                 v.leaf(g.begin, g.end, open);
-                wc.pairs += open ? g.end - g.begin : 0;
+                wc.pairs += nopen * (g.end - g.begin);
                 wc.slots += g.end - g.begin;
             }
             node = skip;
