@@ -204,17 +204,17 @@ static int launch_gravity_leaves(Ctx &c)
 // The pre-interaction's candidate search.  After the first call, one walk
 // finds both the gather candidates and the symmetric pairs the force stage
 // needs (search_combined); the first call keeps the reference's initial
-// smoothing and separate searches.  With walk_after_search the gravity
-// walk is started on the second stream once the search is queued, so that
-// it runs alongside the pre-interaction rather than the search (both the
-// walk and the search are bound by vector-ALU issue).
+// smoothing and separate searches.  With `walk` the gravity walk is started
+// on the second stream just before the search (after the search radii, the
+// cells' largest radius and the search's task counters are set, so that no
+// small kernel of the main stream waits for CUs behind the walk).
 // Kernel-span marks of sphgpu_integrate's timers (ev[8 + k]).
 static void kmark(Ctx &c, int k)
 {
     if(c.kernel_marks) (void)hipEventRecord(c.ev[8 + k], c.stream);
 }
 
-static int stage_pre(Ctx &c, bool walk_after_search = false)
+static int stage_pre(Ctx &c, bool walk = false)
 {
     int rc;
     const bool combined = !c.first_pre;
@@ -229,8 +229,19 @@ static int stage_pre(Ctx &c, bool walk_after_search = false)
     } else {
         if((rc = search_guess(c))) return rc;
         if((rc = tree_set_kernel(c))) return rc;  // largest search radius per cell
+        reset_work(c, 0);                         // the search's task counters
+        // The gravity walk (second stream) runs beside the search.  The
+        // search then takes two blocks of four waves per CU, leaving the
+        // other wave slots to the walk: the search waits on dependent loads
+        // and gains little from more waves, while the walk is bound by
+        // instruction issue (measured per mixed Evrard step of 2^20
+        // particles: 6.71 ms, against 6.83 ms with three blocks per CU and
+        // 7.35 ms with four).
+        const int bpc = walk ? 2 : 8;
+        if(walk && (rc = launch_gravity_walk(c))) return rc;
+        walk = false;
         kmark(c, 0);
-        if((rc = search_combined(c))) return rc;
+        if((rc = search_combined(c, true, bpc))) return rc;
         kmark(c, 1);
     }
     if(c.record) {
@@ -239,7 +250,7 @@ static int stage_pre(Ctx &c, bool walk_after_search = false)
     }
     if(c.need_lists && ((rc = ensure_lists(c)) || (rc = search_expand(c, c.list, c.list_cnt, c.list_cnt2, false))))
         return rc;
-    if(walk_after_search && (rc = launch_gravity_walk(c))) return rc;
+    if(walk && (rc = launch_gravity_walk(c))) return rc;
     kmark(c, 2);
     if((rc = pre_compute(c))) return rc;
     kmark(c, 3);
@@ -628,11 +639,11 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     mark(2);
     if((rc = stage_tree(c))) return rc;
     // Gravity runs on the second stream in two parts: the tree walk (vector-
-    // ALU bound; needs the tree and positions only), started after the
-    // search, alongside the pre-interaction and the force pass (bound by L2
-    // misses on neighbour records), then the recorded leaves' particle pairs
-    // (need h).  The gravity acceleration is added to acc once both streams
-    // are done.
+    // ALU bound; needs the tree and positions only), started once the tree
+    // is built, beside the neighbour search (which waits on dependent loads)
+    // and the pre-interaction; then the recorded leaves' particle pairs
+    // (need h), beside the force pass.  The gravity acceleration is added to
+    // acc once both streams are done.
     const bool grav = c.p.gravity_is_valid;
     mark(3);
     if((rc = stage_pre(c, grav))) {

@@ -18,7 +18,11 @@ constexpr int BLOCK = 256;
 // ------------------------------------------------------------- wave64 helpers
 __device__ __forceinline__ int wave_lane() { return __lane_id(); }
 
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+// Mask of the lanes with p set, in scalar registers (the compare result
+// itself, with no round trip of p through a vector register).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ __forceinline__ bool wave_any(bool p) { return ballot(p) != 0ull; }
 
 __device__ __forceinline__ int wave_min_int(int v)
 {

@@ -327,7 +327,7 @@ int tree_set_kernel(Ctx &c);     // BHTree::set_kernel
 int search_gather(Ctx &c, bool initial);  // h from the density guess
 int search_symmetric(Ctx &c);
 int search_guess(Ctx &c);        // sml = h_guess * ratio
-int search_combined(Ctx &c);     // gather and symmetric candidates in one walk
+int search_combined(Ctx &c, bool work_ready = false, int blocks_per_cu = 8);  // gather and symmetric candidates in one walk
 int search_alloc(Ctx &c);        // tile slabs for c.ngroups groups (after a tree build)
 int search_expand(Ctx &c, int *list, int *cnt, int *cnt2, bool front_only);  // tiles -> list_at lists
 // pre.hip

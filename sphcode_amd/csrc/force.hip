@@ -11,7 +11,7 @@
 // T is the pair arithmetic (double, or float in mixed mode: j fields from
 // FP32 mirrors, separations taken in FP64 then rounded, FP32 terms
 // accumulated in FP64; the pair cut is tested on the FP64 squared distance).
-#include "ctx.hpp"
+#include "tiles.hpp"
 
 namespace sphgpu {
 
@@ -122,10 +122,10 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
     constexpr bool FP64 = sizeof(T) == 8;
     constexpr bool GS = TYPE == SPHGPU_GSPH;
     constexpr int SW = BLOCK / WAVE;
-    __shared__ double4 l_p[SW][WAVE];
-    __shared__ ForceRec<T> l_f[SW][WAVE];
-    __shared__ GradRec<T> l_g[GS ? SW : 1][GS ? WAVE : 1];
-    __shared__ int l_j[SW][WAVE];
+    __shared__ double4 l_p[SW][2][WAVE];
+    __shared__ ForceRec<T> l_f[SW][2][WAVE];
+    __shared__ GradRec<T> l_g[GS ? SW : 1][GS ? 2 : 1][GS ? WAVE : 1];
+    __shared__ int l_j[SW][2][WAVE];
     const int w = threadIdx.x / WAVE;
     const int lane = wave_lane();
     const int grp = xcd_block() * SW + w;
@@ -170,24 +170,22 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         const size_t gbase = (size_t)grp * a.wcap;
         const int nc = a.gnc[grp];
         const int ntile = (nc + WAVE - 1) / WAVE;
-        for(int tt = 0; tt < ntile; ++tt) {
+        auto stage = [&](int tt, int slot) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
-            // stage the tile's records: lane k loads candidate k's
+            // lane k loads candidate k's records
             const int jl = a.cand[o + lane];
             if(jl >= 0) {
-                l_p[w][lane] = a.posrec[jl];
-                l_f[w][lane] = frec[jl];
+                l_p[w][slot][lane] = a.posrec[jl];
+                l_f[w][slot][lane] = frec[jl];
                 if constexpr(GS)
-                    if(a.second_order) l_g[w][lane] = ((const GradRec<T> *)a.grec)[jl];
+                    if(a.second_order) l_g[GS ? w : 0][GS ? slot : 0][lane] = ((const GradRec<T> *)a.grec)[jl];
             }
-            l_j[w][lane] = jl;
-            uint64_t mm = valid ? (a.mf[o + lane] | a.mb[o + lane]) : 0ull;
-            wave_lds_sync();
-            while(mm) {
-            const int kc = __builtin_ctzll(mm);
-            mm &= mm - 1ull;
-            const int j = l_j[w][kc];
-            const double4 pj4 = l_p[w][kc];
+            l_j[w][slot][lane] = jl;
+            return valid ? (a.mf[o + lane] | a.mb[o + lane]) : 0ull;
+        };
+        auto pair = [&](int slot, int kc) {
+            const int j = l_j[w][slot][kc];
+            const double4 pj4 = l_p[w][slot][kc];
             double xj[D], d[D];
             xj[0] = pj4.x;
             if constexpr(D > 1) xj[1] = pj4.y;
@@ -198,9 +196,9 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             T r;
             if constexpr(FP64) {
                 r = sqrt(r2);
-                if(r >= hmax || r == 0.0) continue;  // src/fluid_force.cpp:62-64
+                if(r >= hmax || r == 0.0) return;  // src/fluid_force.cpp:62-64
             } else {
-                if(!(r2 < hmax * hmax) || r2 == 0.0) continue;
+                if(!(r2 < hmax * hmax) || r2 == 0.0) return;
                 r = sqrtf((float)r2);
             }
             if(a.rec && used < a.cap) a.rec[list_at(i, used, a.cap)] = j;
@@ -208,7 +206,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             T rij[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) rij[l] = (T)d[l];
-            const ForceRec<T> fj = l_f[w][kc];
+            const ForceRec<T> fj = l_f[w][slot][kc];
             const Kern kj(T(0), fj.hinv);  // the kernel object reads 1/h only
             const T cwi = ki.dw_coef(ki.q_of(r), r);
             const T cwj = kj.dw_coef(kj.q_of(r), r);
@@ -238,7 +236,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     const T di = T(0.5) * (T(1) - ci * dt * rinv);
                     const T dj = T(0.5) * (T(1) - cj * dt * rinv);
                     const T dv = ve_i - ve_j;
-                    const GradRec<T> &grj = l_g[GS ? w : 0][GS ? kc : 0];
+                    const GradRec<T> &grj = l_g[GS ? w : 0][GS ? slot : 0][GS ? kc : 0];
                     T gvi[D], gvj[D];
 #pragma unroll
                     for(int c = 0; c < D; ++c) {
@@ -331,9 +329,8 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 for(int l = 0; l < D; ++l) acc[l].add(-(dwi[l] * ci_f + dwj[l] * cj_f));
                 dene.add(de + T(0.5) * mj * pij * vdwij + ac);
             }
-            }
-            wave_lds_sync();  // the next tile overwrites the records
-        }
+        };
+        tile_sweep(ntile, stage, pair);
         if(valid) {
 #pragma unroll
             for(int k = 0; k < D; ++k) s.f[F_A + k][i] = acc[k].val();

@@ -128,7 +128,7 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
     __device__ __forceinline__ void leaf(int b, int e, bool open)
     {
         if constexpr(RECORD) {
-            const uint64_t m = uniform64(__ballot(open));
+            const uint64_t m = ballot(open);
             if(nrec < LEAFHIT_CAP && wave_lane() == 0) {
                 LeafHit h;
                 h.b = b;
@@ -281,7 +281,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
                 srec[w][k] = make_int2(h.b, h.e);
             }
             for(int l = 0; l < WAVE; ++l) {
-                const uint64_t col = uniform64(__ballot((m >> l) & 1ull));
+                const uint64_t col = ballot(((m >> l) & 1ull) != 0);
                 if(lane == l) mine[c] = col;
             }
         }
@@ -309,7 +309,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
                 e = be.y;
             }
         }
-        if(uniform64(__ballot(live)) == 0) break;
+        if(ballot(live) == 0) break;
         // two consecutive particles of the leaf per pass (one if it has one
         // left), so that the leaf bookkeeping above runs half as often
         const bool two = live && j + 1 < e;

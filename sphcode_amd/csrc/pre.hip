@@ -24,7 +24,7 @@
 // taken in FP64, then rounded) and accumulated in FP64 (Acc in ctx.hpp),
 // and the Newton update, its convergence test and every stored result
 // stay FP64.
-#include "ctx.hpp"
+#include "tiles.hpp"
 
 namespace sphgpu {
 
@@ -114,7 +114,7 @@ template <int D, int K, class T>
 __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box, PreArgs a)
 {
     using Kern = typename KernelOf<D, K, T>::type;
-    __shared__ PosMass<T> l_r[PSW][WAVE];
+    __shared__ PosMass<T> l_r[PSW][2][WAVE];
     const int w = threadIdx.x / WAVE;
     const int lane = wave_lane();
     const int grp = xcd_block() * PSW + w;
@@ -131,27 +131,24 @@ __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box,
     Acc<T> rho;
     const size_t gbase = (size_t)grp * a.wcap;
     const int ntile = (a.gnc[grp] + WAVE - 1) / WAVE;
-    for(int tt = 0; tt < ntile; ++tt) {
+    auto stage = [&](int tt, int slot) -> uint64_t {
         const size_t o = gbase + (size_t)tt * WAVE;
         const int jl = a.cand[o + lane];
         if(jl >= 0) {
 #pragma unroll
-            for(int k = 0; k < D; ++k) l_r[w][lane].x[k] = s.f[F_X + k][jl];
-            l_r[w][lane].m = fld<T>(s, F_MASS, jl);
+            for(int k = 0; k < D; ++k) l_r[w][slot][lane].x[k] = s.f[F_X + k][jl];
+            l_r[w][slot][lane].m = fld<T>(s, F_MASS, jl);
         }
-        uint64_t mm = valid ? a.mf[o + lane] : 0ull;
-        wave_lds_sync();
-        while(mm) {
-            const int kc = __builtin_ctzll(mm);
-            mm &= mm - 1ull;
-            double xj[D], d[D];
+        return valid ? a.mf[o + lane] : 0ull;
+    };
+    auto pair = [&](int slot, int kc) {
+        double xj[D], d[D];
 #pragma unroll
-            for(int k = 0; k < D; ++k) xj[k] = l_r[w][kc].x[k];
-            const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
-            if(r < h) rho.add(l_r[w][kc].m * ker.w(ker.q_of(r)));
-        }
-        wave_lds_sync();
-    }
+        for(int k = 0; k < D; ++k) xj[k] = l_r[w][slot][kc].x[k];
+        const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
+        if(r < h) rho.add(l_r[w][slot][kc].m * ker.w(ker.q_of(r)));
+    };
+    tile_sweep(ntile, stage, pair);
     if(valid) s.f[F_DENS][i] = rho.val();
 }
 
@@ -160,8 +157,8 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool NUMBER = TYPE == SPHGPU_DISPH;
-    __shared__ PreRec<T> l_r[PSW][WAVE];
-    __shared__ int l_j[PSW][WAVE];
+    __shared__ PreRec<T> l_r[PSW][2][WAVE];
+    __shared__ int l_j[PSW][2][WAVE];
     const int w = threadIdx.x / WAVE;
     const int lane = wave_lane();
     // waves past the last group take part in the block's reduction only
@@ -192,38 +189,35 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
             bool done = !valid;
             int it = 0;
             for(; it < 10; ++it) {
-                if(uniform64(__ballot(!done)) == 0) break;
+                if(ballot(!done) == 0) break;
                 const double hb = hn;
                 const T ht = (T)hn;
                 const Kern ker(ht);
                 Acc<T> rho, drho;
-                for(int tt = 0; tt < ntile; ++tt) {
+                auto nstage = [&](int tt, int slot) -> uint64_t {
                     const size_t o = gbase + (size_t)tt * WAVE;
                     const int jl = a.cand[o + lane];
-                    if(jl >= 0) l_r[w][lane] = prec[jl];
-                    uint64_t mm = done ? 0ull : a.mf[o + lane];
-                    wave_lds_sync();
-                    while(mm) {
-                        const int kc = __builtin_ctzll(mm);
-                        mm &= mm - 1ull;
-                        double xj[D], d[D];
+                    if(jl >= 0) l_r[w][slot][lane] = prec[jl];
+                    return done ? 0ull : a.mf[o + lane];
+                };
+                auto npair = [&](int slot, int kc) {
+                    double xj[D], d[D];
 #pragma unroll
-                        for(int k = 0; k < D; ++k) xj[k] = l_r[w][kc].x[k];
-                        const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
-                        if(r < ht) {
-                            const T q = ker.q_of(r);
-                            if constexpr(NUMBER) {
-                                rho.add(ker.w(q));
-                                drho.add(ker.dhw(q));
-                            } else {
-                                const T mj = l_r[w][kc].m;
-                                rho.add(mj * ker.w(q));
-                                drho.add(mj * ker.dhw(q));
-                            }
+                    for(int k = 0; k < D; ++k) xj[k] = l_r[w][slot][kc].x[k];
+                    const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
+                    if(r < ht) {
+                        const T q = ker.q_of(r);
+                        if constexpr(NUMBER) {
+                            rho.add(ker.w(q));
+                            drho.add(ker.dhw(q));
+                        } else {
+                            const T mj = l_r[w][slot][kc].m;
+                            rho.add(mj * ker.w(q));
+                            drho.add(mj * ker.dhw(q));
                         }
                     }
-                    wave_lds_sync();
-                }
+                };
+                tile_sweep(ntile, nstage, npair);
                 if(!done) {
                     ++iters;
                     const double f = rho.val() * pow_dim<D>(hn) - b;
@@ -258,23 +252,21 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
         const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
-        for(int tt = 0; tt < ntile; ++tt) {
+        auto fstage = [&](int tt, int slot) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
             const int jl = a.cand[o + lane];
-            if(jl >= 0) l_r[w][lane] = prec[jl];
-            l_j[w][lane] = jl;
-            uint64_t mm = valid ? a.mf[o + lane] : 0ull;
-            wave_lds_sync();
-            while(mm) {
-            const int kc = __builtin_ctzll(mm);
-            mm &= mm - 1ull;
-            const PreRec<T> pj = l_r[w][kc];
+            if(jl >= 0) l_r[w][slot][lane] = prec[jl];
+            l_j[w][slot][lane] = jl;
+            return valid ? a.mf[o + lane] : 0ull;
+        };
+        auto fpair = [&](int slot, int kc) {
+            const PreRec<T> pj = l_r[w][slot][kc];
             double xj[D], d[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) xj[l] = pj.x[l];
             const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
-            if(!(r < ht)) continue;
-            const int j = l_j[w][kc];
+            if(!(r < ht)) return;
+            const int j = l_j[w][slot][kc];
             ++count;
             const T mj = pj.m;
             const T q = ker.q_of(r);
@@ -333,9 +325,8 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                     rot.add(vij, dw, wgt);
                 }
             }
-            }
-            wave_lds_sync();  // the next tile overwrites the records
-        }
+        };
+        tile_sweep(ntile, fstage, fpair);
         if(valid) {
             pairs = count;
             const double gm1 = a.gamma - 1.0;

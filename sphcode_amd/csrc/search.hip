@@ -184,9 +184,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             const int incl = wave_incl_max_nonneg(closed_skip);
             const int excl = wave_shr1(incl);
             const bool vis = in && v >= excl;
-            visited += __popcll(__ballot(vis));
+            visited += __popcll(ballot(vis));
             const bool hit = vis && leafhit;
-            const uint64_t hits = __ballot(hit);
+            const uint64_t hits = ballot(hit);
             if(hit) {
                 const int at = nleaf + __popcll(hits & ((1ull << lane) - 1ull));
                 l_begin[w][at] = r.x;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
                 }
                 keep = d2b <= Hc * Hc * (1.0 + 1e-9);
             }
-            const uint64_t kept = __ballot(keep);
+            const uint64_t kept = ballot(keep);
             if(keep) {
                 const int at = nkept + __popcll(kept & ((1ull << lane) - 1ull));
                 if(at < a.wcap) {
@@ -389,24 +389,24 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     }  // groups
 }
 
-template <int D, bool IJ> static void launch_search(Ctx &c, const SearchArgs &a)
+template <int D, bool IJ> static void launch_search(Ctx &c, const SearchArgs &a, bool work_ready, int blocks_per_cu)
 {
     const BoxDev box = make_box(c);
-    // enough blocks to fill every CU (at most 8 blocks of 4 waves each), a
-    // multiple of the XCD count; blocks beyond that find no task and leave
-    const int blocks = std::max(NUM_XCD, std::min(grid_for(c.n, WAVE) / SW, 8 * c.num_cu) / NUM_XCD * NUM_XCD);
-    reset_work(c, 0);
+    // blocks_per_cu blocks of 4 waves per CU (8 fill every CU), a multiple
+    // of the XCD count; blocks beyond what the CUs hold find no task and leave
+    const int blocks = std::max(NUM_XCD, std::min(grid_for(c.n, WAVE) / SW, blocks_per_cu * c.num_cu) / NUM_XCD * NUM_XCD);
+    if(!work_ready) reset_work(c, 0);
     SearchArgs aw = a;
     aw.work = c.work;
     hipLaunchKernelGGL((search_kernel<D, IJ>), dim3(blocks), dim3(BLOCK), 0, c.stream, c.s, c.t, box, c.groups, c.num_groups, aw);
 }
 
-template <bool IJ> static int search_dispatch(Ctx &c, const SearchArgs &a)
+template <bool IJ> static int search_dispatch(Ctx &c, const SearchArgs &a, bool work_ready = false, int blocks_per_cu = 8)
 {
     switch(c.D) {
-    case 1: launch_search<1, IJ>(c, a); break;
-    case 2: launch_search<2, IJ>(c, a); break;
-    default: launch_search<3, IJ>(c, a); break;
+    case 1: launch_search<1, IJ>(c, a, work_ready, blocks_per_cu); break;
+    case 2: launch_search<2, IJ>(c, a, work_ready, blocks_per_cu); break;
+    default: launch_search<3, IJ>(c, a, work_ready, blocks_per_cu); break;
     }
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
@@ -453,13 +453,16 @@ static SearchArgs base_args(Ctx &c)
 // sml from tree_set_kernel, the test r < max(sml_i, sml_j) finds every
 // gather candidate (r < sml_i, in the front masks) and every pair with
 // r < max(h_i, h_j) for the final h, as long as no final h exceeds its
-// search radius (the caller checks the truncation counter).
-int search_combined(Ctx &c)
+// search radius (the caller checks the truncation counter).  With
+// `work_ready` the caller has already cleared the task counters
+// (reset_work(c, 0)); `blocks_per_cu` sets how many of the CUs' wave slots
+// the search's persistent waves take (8: all of them).
+int search_combined(Ctx &c, bool work_ready, int blocks_per_cu)
 {
     SearchArgs a = base_args(c);
     a.split = 1;
     a.guess = 0;
-    return search_dispatch<true>(c, a);
+    return search_dispatch<true>(c, a, work_ready, blocks_per_cu);
 }
 
 int search_gather(Ctx &c, bool initial)

@@ -49,7 +49,7 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
 {
     WalkCount wc;
     const int lane = wave_lane();
-    uint64_t active = uniform64(__ballot(valid));
+    uint64_t active = ballot(valid);
     int node = 0;
     int sp = 0;                    // entries below the top, in LDS
     int top_node = WALK_DONE;      // top entry
@@ -81,7 +81,7 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
         const GravNode g = load_ro(reinterpret_cast<const GravNode *>(gnb + ((uint32_t)node << 6)));
         const int skip = g.skip;
         const bool open = v.node(g, me);
-        const uint64_t om = uniform64(__ballot(open));
+        const uint64_t om = ballot(open);
         const int nopen = __popcll(om);
         wc.opened += nopen;
         wc.accepted += __popcll(active) - nopen;
