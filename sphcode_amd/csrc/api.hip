@@ -235,7 +235,7 @@ static int stage_pre(Ctx &c, bool walk = false)
         // other wave slots to the walk: the search waits on dependent loads
         // and gains little from more waves, while the walk is bound by
         // instruction issue (measured per mixed Evrard step of 2^20
-        // particles: 6.71 ms, against 6.83 ms with three blocks per CU and
+        // particles: 7.04 ms, against 7.16 ms with three blocks per CU and
         // 7.35 ms with four).
         const int bpc = walk ? 2 : 8;
         if(walk && (rc = launch_gravity_walk(c))) return rc;

@@ -24,6 +24,28 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 
 __device__ __forceinline__ bool wave_any(bool p) { return ballot(p) != 0ull; }
 
+// 64 x 64 bit-matrix transpose across the wave: lane l returns the word
+// whose bit k is bit l of lane k's word.  Six butterfly stages: at block
+// size s, lanes l and l ^ s swap the s-bit blocks that lie across the
+// diagonal.  Every lane of the wave must take part.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
+{
+    const int lane = wave_lane();
+    constexpr uint64_t M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for(int st = 0; st < 6; ++st) {
+        const int s = 32 >> st;
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)a, s, WAVE);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(a >> 32), s, WAVE);
+        const uint64_t p = ((uint64_t)hi << 32) | lo;
+        const bool up = (lane & s) != 0;
+        const uint64_t t = (((up ? p : a) >> s) ^ (up ? a : p)) & M[st];
+        a ^= up ? t : (t << s);
+    }
+    return a;
+}
+
 __device__ __forceinline__ int wave_min_int(int v)
 {
     for(int off = 32; off > 0; off >>= 1) {

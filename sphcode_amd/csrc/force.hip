@@ -330,7 +330,24 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 dene.add(de + T(0.5) * mj * pij * vdwij + ac);
             }
         };
-        tile_sweep(ntile, stage, pair);
+        // the cut-off r < max(h_i, h_j), r > 0 of the pair, tested ahead
+        auto keep = [&](int slot, int kc) -> bool {
+            const double4 pj4 = l_p[w][slot][kc];
+            double xj[D], d[D];
+            xj[0] = pj4.x;
+            if constexpr(D > 1) xj[1] = pj4.y;
+            if constexpr(D > 2) xj[2] = pj4.z;
+            const double r2 = sep2<D>(box, xi, xj, d);
+            const double hmax = hi_d < pj4.w ? pj4.w : hi_d;
+            if constexpr(FP64) {
+                const double r = sqrt(r2);
+                return !(r >= hmax || r == 0.0);
+            } else {
+                return r2 < hmax * hmax && r2 != 0.0;
+            }
+        };
+        auto refine = [&](int slot, uint64_t m) { return refine_bits(slot, m, keep); };
+        tile_sweep(ntile, stage, refine, pair);
         if(valid) {
 #pragma unroll
             for(int k = 0; k < D; ++k) s.f[F_A + k][i] = acc[k].val();

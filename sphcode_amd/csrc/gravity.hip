@@ -280,10 +280,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
                 m = h.mask;
                 srec[w][k] = make_int2(h.b, h.e);
             }
-            for(int l = 0; l < WAVE; ++l) {
-                const uint64_t col = ballot(((m >> l) & 1ull) != 0);
-                if(lane == l) mine[c] = col;
-            }
+            mine[c] = wave_transpose64(m);
         }
     }
     wave_lds_sync();

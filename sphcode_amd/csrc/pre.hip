@@ -326,7 +326,15 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 }
             }
         };
-        tile_sweep(ntile, fstage, fpair);
+        // the cut-off r < h of the final sums, tested ahead
+        auto fkeep = [&](int slot, int kc) -> bool {
+            double xj[D], d[D];
+#pragma unroll
+            for(int l = 0; l < D; ++l) xj[l] = l_r[w][slot][kc].x[l];
+            return gather_r<T>(sep2<D>(box, xi, xj, d)) < ht;
+        };
+        auto frefine = [&](int slot, uint64_t m) { return refine_bits(slot, m, fkeep); };
+        tile_sweep(ntile, fstage, frefine, fpair);
         if(valid) {
             pairs = count;
             const double gm1 = a.gamma - 1.0;
