@@ -24,6 +24,10 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 
 __device__ __forceinline__ bool wave_any(bool p) { return ballot(p) != 0ull; }
 
+// This lane's bit of a wave-uniform lane mask, as a condition the vector
+// instructions take straight from the scalar registers.
+__device__ __forceinline__ bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
+
 // 64 x 64 bit-matrix transpose across the wave: lane l returns the word
 // whose bit k is bit l of lane k's word.  Six butterfly stages: at block
 // size s, lanes l and l ^ s swap the s-bit blocks that lie across the

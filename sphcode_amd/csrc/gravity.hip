@@ -96,11 +96,12 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
 
     __device__ __forceinline__ GravityVisitor(const BoxDev &b_, const GravPart<T> *gp_) : box(b_), gp(gp_) {}
 
-    // Opening test edge^2 > theta^2 |x_i - com|^2 (FP64, all lanes); a lane
-    // of `me` that does not open adds the monopole, the others add zero.
-    // |x_i - com|^2 is summed left to right without contraction, as the
-    // reference's inner product is, so both take the same cells.
-    __device__ __forceinline__ bool node(const GravNode &g, bool me)
+    // Opening test edge^2 > theta^2 |x_i - com|^2 (FP64, all lanes); the
+    // lanes of `act` that do not open the node add its monopole, the others
+    // add zero.  |x_i - com|^2 is summed left to right without contraction,
+    // as the reference's inner product is, so both take the same cells.
+    // Returns the lanes of `act` that open the node.
+    __device__ __forceinline__ uint64_t node(const GravNode &g, uint64_t act)
     {
 #pragma clang fp contract(off)
         double d[D];
@@ -111,8 +112,8 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
         double d2 = d[0] * d[0];
 #pragma unroll
         for(int k = 1; k < D; ++k) d2 = d2 + d[k] * d[k];
-        const bool o = g.edge2 > theta2 * d2;
-        const bool far = me && !o;
+        const uint64_t om = act & ballot(g.edge2 > theta2 * d2);
+        const bool far = lane_in(act & ~om);
         T gmn;
         if constexpr(sizeof(T) == 4) gmn = g.gmf;
         else gmn = g.gm;
@@ -122,24 +123,23 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
         const T c = gm * rinv * rinv * rinv;
 #pragma unroll
         for(int k = 0; k < D; ++k) acc[k].add_prod(d[k], -c);
-        return me && o;
+        return om;
     }
 
-    __device__ __forceinline__ void leaf(int b, int e, bool open)
+    __device__ __forceinline__ void leaf(int b, int e, uint64_t open)
     {
         if constexpr(RECORD) {
-            const uint64_t m = ballot(open);
             if(nrec < LEAFHIT_CAP && wave_lane() == 0) {
                 LeafHit h;
                 h.b = b;
                 h.e = e;
-                h.mask = m;
+                h.mask = open;
                 rec[nrec] = h;
             }
             ++nrec;
             return;
         }
-        const T w = open ? T(1) : T(0);
+        const T w = lane_in(open) ? T(1) : T(0);
         for(int j = b; j < e; ++j) pair(load_ro(gp + j), w);
     }
 

@@ -24,20 +24,25 @@ constexpr int WALK_DONE = 0x7fffffff;
 constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 
 // Visitor interface (every call is made by the whole wave):
-//   bool node(const GravNode &g, bool me) the lane's own decision at the
-//                                        node, for lanes with `me`; the
-//                                        visitor sums a node accepted without
-//                                        opening itself, masked rather than
-//                                        branched
-//   void leaf(int b, int e, bool open)   particles [b, e) of an opened leaf,
-//                                        for the lanes with `open`
-// The walk state (cursor, masks, parked stack) is wave-uniform and kept in
-// scalar registers; each node is one 64-byte record (GravNode) read with
-// one scalar load.  WalkCount returns the wave's node visits, the
-// particles of the leaves that some lane opened, and the lanes' opened
-// nodes, accepted nodes and particle interactions, all wave totals counted
-// from the lane masks with scalar bit counts (no vector instructions per
-// visit).
+//   uint64_t node(const GravNode &g, uint64_t act)
+//                                        the opening decisions at the node of
+//                                        the lanes in `act` (the lanes whose
+//                                        traversal is there), as the mask of
+//                                        the lanes that open it; the visitor
+//                                        sums the node for the lanes of `act`
+//                                        that do not open it, masked rather
+//                                        than branched
+//   void leaf(int b, int e, uint64_t open)
+//                                        particles [b, e) of an opened leaf,
+//                                        for the lanes of `open`
+// The walk state (cursor, lane masks, parked stack) is wave-uniform and kept
+// in scalar registers; lanes take their part of a mask with lane_in, so no
+// per-lane flag is carried from one visit to the next.  Each node is one
+// 64-byte record (GravNode) read with one scalar load.  WalkCount returns
+// the wave's node visits, the particles of the leaves that some lane
+// opened, and the lanes' opened nodes, accepted nodes and particle
+// interactions, all wave totals counted from the lane masks with scalar
+// bit counts (no vector instructions per visit).
 struct WalkCount {
     int visits = 0, slots = 0;
     int opened = 0, accepted = 0, pairs = 0;
@@ -48,7 +53,6 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
                                                uint64_t *stk_mask)
 {
     WalkCount wc;
-    const int lane = wave_lane();
     uint64_t active = ballot(valid);
     int node = 0;
     int sp = 0;                    // entries below the top, in LDS
@@ -75,24 +79,22 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
     // stay in the same registers from one visit to the next.
     if(active == 0 || total <= 0) return wc;
     const char *gnb = reinterpret_cast<const char *>(gn);
-    bool me = valid;  // this lane's bit of `active`, kept alongside it
     do {
         ++wc.visits;
         const GravNode g = load_ro(reinterpret_cast<const GravNode *>(gnb + ((uint32_t)node << 6)));
         const int skip = g.skip;
-        const bool open = v.node(g, me);
-        const uint64_t om = ballot(open);
+        const uint64_t om = v.node(g, active);
         const int nopen = __popcll(om);
         wc.opened += nopen;
         wc.accepted += __popcll(active) - nopen;
-        if(g.leaf) {
-            if(om) {
-                v.leaf(g.begin, g.end, open);
-                wc.pairs += nopen * (g.end - g.begin);
-                wc.slots += g.end - g.begin;
-            }
+        if(om == 0) {
             node = skip;
-        } else if(om) {
+        } else if(g.leaf) {
+            v.leaf(g.begin, g.end, om);
+            wc.pairs += nopen * (g.end - g.begin);
+            wc.slots += g.end - g.begin;
+            node = skip;
+        } else {
             const uint64_t fm = active & ~om;
             if(fm) {
                 // park the lanes that accepted this node at its skip index
@@ -109,14 +111,10 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
                 }
             }
             active = om;
-            me = open;
             node = node + 1;
-        } else {
-            node = skip;
         }
         while(node == top_node) {
             active |= top_mask;
-            me = me || ((top_mask >> lane) & 1ull);
             pop();
         }
     } while(node < total);
