@@ -168,6 +168,9 @@ typedef struct sphgpu_stats {
                                      full, per particle of each leaf that one
                                      of its lanes opened); gravity_pp / this
                                      is the pass's lane efficiency             */
+    int64_t gravity_max_records;  /* most leaves one 64-particle group opened
+                                     (records above 1024 per group fall back
+                                     to the full walk)                         */
 } sphgpu_stats;
 
 /* Stage timers (milliseconds, HIP events on the context stream),

@@ -99,7 +99,7 @@ class Stats(C.Structure):
         "newton_failures", "newton_iterations", "search_truncations",
         "list_overflows", "max_list_length", "tree_nodes", "tree_max_level", "gravity_opened",
         "search_nodes", "search_candidates", "gravity_wave_steps", "gravity_rewalks",
-        "gravity_pp_slots")]
+        "gravity_pp_slots", "gravity_max_records")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

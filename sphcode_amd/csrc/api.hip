@@ -880,6 +880,7 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
     st->gravity_wave_steps = k[C_GRAV_STEPS];
     st->gravity_rewalks = k[C_GRAV_REWALK];
     st->gravity_pp_slots = k[C_GRAV_SLOTS];
+    st->gravity_max_records = k[C_GRAV_MAXREC];
     st->newton_failures = k[C_NEWTON_FAIL];
     st->newton_iterations = k[C_NEWTON_ITERS];
     st->search_truncations = k[C_TRUNC];

@@ -84,7 +84,8 @@ struct __align__(16) LeafHit {
     int b, e;
     uint64_t mask;
 };
-constexpr int LEAFHIT_CAP = 256;  // recorded leaves per wave before it falls back
+constexpr int LEAFHIT_CAP = 1024;   // recorded leaves per wave before it falls back
+constexpr int LEAFHIT_CHUNK = 256;  // records the pair pass takes per round
 template <class T> struct GravPart;
 template <> struct __align__(32) GravPart<float> {  // mixed: FP32 mirrors of m and 2/h
     double x[3];
@@ -166,7 +167,7 @@ __device__ __forceinline__ int list_slot(int kk, int front, int cap) { return kk
 enum Counter : int {
     C_GATHER_PAIRS = 0, C_FORCE_PAIRS, C_GRAV_PP, C_GRAV_PN, C_NEWTON_FAIL, C_NEWTON_ITERS,
     C_TRUNC, C_OVERFLOW, C_MAXLIST, C_TREE_ERR, C_GRAV_OPEN, C_SEARCH_NODES, C_SEARCH_CAND, C_GRAV_STEPS, C_GRAV_REWALK, C_GRAV_SLOTS,
-    C_TILE_OVER, NCOUNTERS
+    C_TILE_OVER, C_GRAV_MAXREC, NCOUNTERS
 };
 
 // Device scalars.

@@ -209,6 +209,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     if(wave_lane() == 0) {
         a.leafcnt[task] = v.nrec;
         if(v.nrec > LEAFHIT_CAP) atomicAdd((unsigned long long *)&a.counters[C_GRAV_REWALK], 1ull);
+        atomicMax((unsigned long long *)&a.counters[C_GRAV_MAXREC], (unsigned long long)(unsigned)v.nrec);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)pairs);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)accepted);
         atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)opened);
@@ -260,62 +261,67 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     // every lane steps through the (leaf, particle) sequence of its set
     // bits at its own pace, so no lane waits on leaves that it did not
     // open.  A lane adds its pairs in the same order as a sweep over all
-    // records would (records in walk order, particles in tree order).
+    // records would (records in walk order, particles in tree order).  The
+    // records are taken LEAFHIT_CHUNK at a time, in order.
     const LeafHit *rec = a.leafhit + (int64_t)task * LEAFHIT_CAP;
     const int lane = wave_lane();
     const int w = uniform(threadIdx.x / WAVE);
-    __shared__ int2 srec[BLOCK / WAVE][LEAFHIT_CAP];
-    constexpr int NW = LEAFHIT_CAP / WAVE;
+    __shared__ int2 srec[BLOCK / WAVE][LEAFHIT_CHUNK];
+    constexpr int NW = LEAFHIT_CHUNK / WAVE;
     static_assert(NW == 4, "four record words per lane");
-    const int nwords = (nrec + WAVE - 1) / WAVE;
-    uint64_t mine[NW];
-#pragma unroll
-    for(int c = 0; c < NW; ++c) {
-        mine[c] = 0;
-        if(c < nwords) {
-            const int k = c * WAVE + lane;
-            uint64_t m = 0;
-            if(k < nrec) {
-                const LeafHit h = rec[k];
-                m = h.mask;
-                srec[w][k] = make_int2(h.b, h.e);
-            }
-            mine[c] = wave_transpose64(m);
-        }
-    }
-    wave_lds_sync();
     const GravPart<T> *gp = (const GravPart<T> *)a.gpart;
-    int c = 0, j = 0, e = 0;
-    uint64_t cur = mine[0];
-    bool live = true;
     int iters = 0;
-    while(true) {
-        // next leaf of this lane once its current one is done
-        while(live && j == e) {
-            while(cur == 0 && c + 1 < nwords) {
-                ++c;
-                cur = c == 1 ? mine[1] : (c == 2 ? mine[2] : mine[NW - 1]);
-            }
-            if(cur == 0) {
-                live = false;
-            } else {
-                const int r = (int)__builtin_ctzll(cur);
-                cur &= cur - 1;
-                const int2 be = srec[w][c * WAVE + r];
-                j = be.x;
-                e = be.y;
+    for(int base = 0; base < nrec; base += LEAFHIT_CHUNK) {
+        const int cnt = min(nrec - base, LEAFHIT_CHUNK);
+        const int nwords = (cnt + WAVE - 1) / WAVE;
+        uint64_t mine[NW];
+#pragma unroll
+        for(int c = 0; c < NW; ++c) {
+            mine[c] = 0;
+            if(c < nwords) {
+                const int k = c * WAVE + lane;
+                uint64_t m = 0;
+                if(k < cnt) {
+                    const LeafHit h = rec[base + k];
+                    m = h.mask;
+                    srec[w][k] = make_int2(h.b, h.e);
+                }
+                mine[c] = wave_transpose64(m);
             }
         }
-        if(ballot(live) == 0) break;
-        // two consecutive particles of the leaf per pass (one if it has one
-        // left), so that the leaf bookkeeping above runs half as often
-        const bool two = live && j + 1 < e;
-        const GravPart<T> p0 = gp[live ? j : 0];
-        const GravPart<T> p1 = gp[two ? j + 1 : 0];
-        v.pair(p0, live ? T(1) : T(0));
-        v.pair(p1, two ? T(1) : T(0));
-        j += two ? 2 : (live ? 1 : 0);
-        ++iters;
+        wave_lds_sync();
+        int c = 0, j = 0, e = 0;
+        uint64_t cur = mine[0];
+        bool live = true;
+        while(true) {
+            // next leaf of this lane once its current one is done
+            while(live && j == e) {
+                while(cur == 0 && c + 1 < nwords) {
+                    ++c;
+                    cur = c == 1 ? mine[1] : (c == 2 ? mine[2] : mine[NW - 1]);
+                }
+                if(cur == 0) {
+                    live = false;
+                } else {
+                    const int r = (int)__builtin_ctzll(cur);
+                    cur &= cur - 1;
+                    const int2 be = srec[w][c * WAVE + r];
+                    j = be.x;
+                    e = be.y;
+                }
+            }
+            if(ballot(live) == 0) break;
+            // two consecutive particles of the leaf per pass (one if it has
+            // one left), so that the leaf bookkeeping above runs half as often
+            const bool two = live && j + 1 < e;
+            const GravPart<T> p0 = gp[live ? j : 0];
+            const GravPart<T> p1 = gp[two ? j + 1 : 0];
+            v.pair(p0, live ? T(1) : T(0));
+            v.pair(p1, two ? T(1) : T(0));
+            j += two ? 2 : (live ? 1 : 0);
+            ++iters;
+        }
+        wave_lds_sync();  // srec is refilled by the next round
     }
     if(lane == 0) atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)(2 * WAVE) * (unsigned)iters);
     if(valid) {

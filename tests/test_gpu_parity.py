@@ -167,15 +167,19 @@ def test_three_steps(gpu, case):
     assert _rms_err(b["acc"], a["acc"]) <= 1e-5, f"{case}: acc err {_rms_err(b['acc'], a['acc']):.3e}"
 
 
-def test_gravity_leaf_record_overflow(gpu):
+@pytest.mark.parametrize("theta", [0.5, 0.2])
+def test_gravity_leaf_record_overflow(gpu, theta):
     """leafParticleNumber 1 (the reference's default, src/solver.cpp:241):
-    every opened leaf is one particle, so the walk's per-wave record of
-    opened leaves (256 entries) overflows and those groups are walked again
-    in full by the leaf pass.  The result must still be the oracle's phi
-    and acceleration, at initialize and after a step (the other gravity
-    cases cover groups whose record fits)."""
+    every opened leaf is one particle, so a 64-particle group opens several
+    hundred leaves.  At theta 0.5 some groups record more than the 256 the
+    pair pass takes per round, so their records are taken in rounds; at
+    theta 0.2 some open more than the 1024 a group can record and are walked
+    again in full by the pair pass.  Either way the result must be the
+    oracle's phi and acceleration, at initialize and after a step (the
+    other gravity cases cover groups whose record fits one round)."""
     from sphcode_amd import ics
     p, f = ics.evrard(4096, leaf=1)
+    p.gravity_theta = theta
     ref = OracleEngine(p)
     from sphcode_amd import GPUEngine
     gpu_e = GPUEngine(p)
@@ -183,8 +187,13 @@ def test_gravity_leaf_record_overflow(gpu):
         e.upload(f)
         e.initialize()
     st = gpu_e.stats()
-    print(f"[overflow] rewalks={st['gravity_rewalks']} groups={len(f['mass']) // 64}", flush=True)
-    assert st["gravity_rewalks"] > 0
+    print(f"[overflow] theta={theta} rewalks={st['gravity_rewalks']} max_records={st['gravity_max_records']} "
+          f"groups={len(f['mass']) // 64}", flush=True)
+    assert st["gravity_max_records"] > 256
+    if theta == 0.5:
+        assert st["gravity_rewalks"] == 0 and st["gravity_max_records"] <= 1024
+    else:
+        assert st["gravity_rewalks"] > 0
     for step in range(2):
         a = ref.download()
         b = gpu_e.download()
