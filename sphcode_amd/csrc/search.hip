@@ -273,12 +273,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
                 const int at = nkept + __popcll(kept & ((1ull << lane) - 1ull));
                 if(at < a.wcap) {
                     a.ws_j[wbase + at] = c_jj;
-                    double4 p;
-                    p.x = c_xj[0];
-                    p.y = D > 1 ? c_xj[1] : 0.0;
-                    p.z = D > 2 ? c_xj[2] : 0.0;
-                    p.w = IJ ? c_lm * c_lm : 0.0;
-                    a.ws_p[wbase + at] = p;
+                    // the position and radius are staged only where phase 3
+                    // cannot read them back from the particle (the leaf's
+                    // largest h of an unsplit symmetric search)
+                    if(IJ && !a.split) a.ws_p[wbase + at] = make_double4(c_xj[0], D > 1 ? c_xj[1] : 0.0, D > 2 ? c_xj[2] : 0.0, c_lm * c_lm);
                 }
             }
             const int m = uniform(__popcll(kept));
@@ -294,17 +292,33 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         wave_lds_sync();  // leaf tables are rewritten by the next batch
     }
 
-    // ---- phase 3: strided tiles of the staged candidates, the next tile's
-    // staged records loaded while the current one is tested
+    // ---- phase 3: strided tiles of the kept candidates, the next tile's
+    // records loaded while the current one is tested
     const int nc = nkept < a.wcap ? nkept : a.wcap;
     const int T = (nc + WAVE - 1) / WAVE;
     // the staging writes of all lanes before their reads by other lanes
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     int p_j = -1;
     double4 p_p = make_double4(0.0, 0.0, 0.0, 0.0);
+    // a kept candidate's position and radius squared: staged, or read back
+    // from the particle arrays (mostly still in L2 from phase 2)
+    const bool staged_p = IJ && !a.split;
+    auto fetch_p = [&](int j) -> double4 {
+        double4 q;
+        q.x = s.f[F_X][j];
+        q.y = D > 1 ? s.f[F_X + 1][j] : 0.0;
+        q.z = D > 2 ? s.f[F_X + 2][j] : 0.0;
+        if constexpr(IJ) {
+            const double l = s.f[F_SML][j];
+            q.w = l * l;
+        } else {
+            q.w = 0.0;
+        }
+        return q;
+    };
     if(T > 0 && lane * T < nc) {
         p_j = a.ws_j[wbase + lane * T];
-        p_p = a.ws_p[wbase + lane * T];
+        p_p = staged_p ? a.ws_p[wbase + lane * T] : fetch_p(p_j);
     }
     for(int tt = 0; tt < T; ++tt) {
         const int m = (nc - tt + T - 1) / T;  // candidates of this tile (a prefix of its lanes)
@@ -321,7 +335,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         p_j = -1;
         if(tt + 1 < T && cn < nc) {
             p_j = a.ws_j[wbase + cn];
-            p_p = a.ws_p[wbase + cn];
+            p_p = staged_p ? a.ws_p[wbase + cn] : fetch_p(p_j);
         }
         wave_lds_sync();
         // every lane tests every candidate of the tile (uniform loop,
