@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         auto stage = [&](int tt, int slot) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
             // lane k loads candidate k's records
-            const int jl = a.cand[o + lane];
+            const int jl = stream_ld(a.cand + o + lane);
             if(jl >= 0) {
                 l_p[w][slot][lane] = a.posrec[jl];
                 l_f[w][slot][lane] = frec[jl];
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     if(a.second_order) l_g[GS ? w : 0][GS ? slot : 0][lane] = ((const GradRec<T> *)a.grec)[jl];
             }
             l_j[w][slot][lane] = jl;
-            return valid ? (a.mf[o + lane] | a.mb[o + lane]) : 0ull;
+            return valid ? (stream_ld(a.mf + o + lane) | stream_ld(a.mb + o + lane)) : 0ull;
         };
         auto pair = [&](int slot, int kc) {
             const int j = l_j[w][slot][kc];

@@ -47,6 +47,7 @@ struct PreArgs {
     int64_t *counters;
     double *partial;      // per-block min of h / v_sig
     const void *prec;     // PreRec<T> per particle
+    const double4 *pm;    // FP64 position and mass per particle (Newton sweeps)
     void *gpart;          // GravPart<T> per particle, written with h (gravity on)
 };
 
@@ -60,14 +61,16 @@ template <class T> __device__ __forceinline__ T gather_r(double r2)
     else return gather_r_f32(r2);
 }
 
-// Records read for j by the final sweep (FP32 mirrors in mixed mode).
-template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_pre_kernel(State s, int n, PreRec<T> *out)
+// Records read for j by the final sweep (FP32 mirrors in mixed mode), and
+// the 32-byte position and mass records the Newton sweeps read.
+template <int D, class T> __global__ void __launch_bounds__(BLOCK) pack_pre_kernel(State s, int n, PreRec<T> *out, double4 *pm)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
     PreRec<T> r{};
 #pragma unroll
     for(int k = 0; k < 3; ++k) r.x[k] = k < D ? s.f[F_X + k][i] : 0.0;
+    pm[i] = make_double4(r.x[0], r.x[1], r.x[2], s.f[F_MASS][i]);
     r.m = fld<T>(s, F_MASS, i);
     r.u = fld<T>(s, F_ENE, i);
     r.c = fld<T>(s, F_SOUND, i);
@@ -133,13 +136,13 @@ __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box,
     const int ntile = (a.gnc[grp] + WAVE - 1) / WAVE;
     auto stage = [&](int tt, int slot) -> uint64_t {
         const size_t o = gbase + (size_t)tt * WAVE;
-        const int jl = a.cand[o + lane];
+        const int jl = stream_ld(a.cand + o + lane);
         if(jl >= 0) {
 #pragma unroll
             for(int k = 0; k < D; ++k) l_r[w][slot][lane].x[k] = s.f[F_X + k][jl];
             l_r[w][slot][lane].m = fld<T>(s, F_MASS, jl);
         }
-        return valid ? a.mf[o + lane] : 0ull;
+        return valid ? stream_ld(a.mf + o + lane) : 0ull;
     };
     auto pair = [&](int slot, int kc) {
         double xj[D], d[D];
@@ -160,6 +163,11 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
     __shared__ PreRec<T> l_r[PSW][2][WAVE];
     __shared__ int l_j[PSW][2][WAVE];
     const int w = threadIdx.x / WAVE;
+    // the Newton sweeps stage 32-byte position and mass records in the same
+    // LDS as the final sweep's records (2 x 64 of them per wave)
+    double4 *const l_pm = reinterpret_cast<double4 *>(&l_r[w][0][0]);
+    static_assert(sizeof(PreRec<T>) >= sizeof(double4) && (2 * WAVE * sizeof(PreRec<T>)) % alignof(double4) == 0,
+                  "position and mass records fit the final sweep's slots");
     const int lane = wave_lane();
     // waves past the last group take part in the block's reduction only
     const int grp = xcd_block() * PSW + w;
@@ -196,14 +204,16 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 Acc<T> rho, drho;
                 auto nstage = [&](int tt, int slot) -> uint64_t {
                     const size_t o = gbase + (size_t)tt * WAVE;
-                    const int jl = a.cand[o + lane];
-                    if(jl >= 0) l_r[w][slot][lane] = prec[jl];
-                    return done ? 0ull : a.mf[o + lane];
+                    const int jl = stream_ld(a.cand + o + lane);
+                    if(jl >= 0) l_pm[slot * WAVE + lane] = a.pm[jl];
+                    return done ? 0ull : stream_ld(a.mf + o + lane);
                 };
                 auto npair = [&](int slot, int kc) {
+                    const double4 pj = l_pm[slot * WAVE + kc];
                     double xj[D], d[D];
-#pragma unroll
-                    for(int k = 0; k < D; ++k) xj[k] = l_r[w][slot][kc].x[k];
+                    xj[0] = pj.x;
+                    if constexpr(D > 1) xj[1] = pj.y;
+                    if constexpr(D > 2) xj[2] = pj.z;
                     const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
                     if(r < ht) {
                         const T q = ker.q_of(r);
@@ -211,7 +221,7 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                             rho.add(ker.w(q));
                             drho.add(ker.dhw(q));
                         } else {
-                            const T mj = l_r[w][slot][kc].m;
+                            const T mj = (T)pj.w;
                             rho.add(mj * ker.w(q));
                             drho.add(mj * ker.dhw(q));
                         }
@@ -254,10 +264,10 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         int count = 0;
         auto fstage = [&](int tt, int slot) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
-            const int jl = a.cand[o + lane];
+            const int jl = stream_ld(a.cand + o + lane);
             if(jl >= 0) l_r[w][slot][lane] = prec[jl];
             l_j[w][slot][lane] = jl;
-            return valid ? a.mf[o + lane] : 0ull;
+            return valid ? stream_ld(a.mf + o + lane) : 0ull;
         };
         auto fpair = [&](int slot, int kc) {
             const PreRec<T> pj = l_r[w][slot][kc];
@@ -452,6 +462,7 @@ static PreArgs make_args(Ctx &c)
     a.counters = c.counters;
     a.partial = c.partial;
     a.prec = c.pair_rec;
+    a.pm = c.posrec;  // free during the pre-interaction (the force pass repacks it)
     return a;
 }
 
@@ -490,7 +501,7 @@ int pre_initial(Ctx &c)
 template <int D, int K, int TYPE, class T> static void launch_pre(Ctx &c, const PreArgs &a, const BoxDev &box)
 {
     hipLaunchKernelGGL((pack_pre_kernel<D, T>), dim3(grid_for(c.n)), dim3(BLOCK), 0, c.stream, c.s, (int)c.n,
-                       (PreRec<T> *)c.pair_rec);
+                       (PreRec<T> *)c.pair_rec, c.posrec);
     hipLaunchKernelGGL((pre_kernel<D, K, TYPE, T>), dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.s, box, (int)c.n, a);
 }
 

@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             if constexpr(D > 2) c_x[w][2][lane] = p_p.z;
             if constexpr(IJ) c_lim[w][lane] = p_p.w;
         }
-        a.cand[gbase + (size_t)tt * WAVE + lane] = j;
+        stream_st(a.cand + gbase + (size_t)tt * WAVE + lane, j);
         // next tile's records
         const int cn = tt + 1 + lane * T;
         p_j = -1;
@@ -376,8 +376,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             mfront = a.split ? ((uint64_t)fhi << 32) | flo : mine;
         }
         const size_t mo = gbase + (size_t)tt * WAVE + lane;
-        a.mf[mo] = mfront;
-        a.mb[mo] = mine & ~mfront;
+        stream_st(a.mf + mo, mfront);
+        stream_st(a.mb + mo, mine & ~mfront);
         count += __popcll(mfront);
         count2 += __popcll(mine & ~mfront);
         wave_lds_sync();
