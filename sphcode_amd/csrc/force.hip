@@ -122,10 +122,10 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
     constexpr bool FP64 = sizeof(T) == 8;
     constexpr bool GS = TYPE == SPHGPU_GSPH;
     constexpr int SW = BLOCK / WAVE;
-    __shared__ double4 l_p[SW][2][WAVE];
-    __shared__ ForceRec<T> l_f[SW][2][WAVE];
-    __shared__ GradRec<T> l_g[GS ? SW : 1][GS ? 2 : 1][GS ? WAVE : 1];
-    __shared__ int l_j[SW][2][WAVE];
+    __shared__ double4 l_p[SW][WAVE];
+    __shared__ ForceRec<T> l_f[SW][WAVE];
+    __shared__ GradRec<T> l_g[GS ? SW : 1][GS ? WAVE : 1];
+    __shared__ int l_j[SW][WAVE];
     const int w = threadIdx.x / WAVE;
     const int lane = wave_lane();
     const int grp = xcd_block() * SW + w;
@@ -170,22 +170,22 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         const size_t gbase = (size_t)grp * a.wcap;
         const int nc = a.gnc[grp];
         const int ntile = (nc + WAVE - 1) / WAVE;
-        auto stage = [&](int tt, int slot) -> uint64_t {
+        auto stage = [&](int tt) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
             // lane k loads candidate k's records
             const int jl = stream_ld(a.cand + o + lane);
             if(jl >= 0) {
-                l_p[w][slot][lane] = a.posrec[jl];
-                l_f[w][slot][lane] = frec[jl];
+                l_p[w][lane] = a.posrec[jl];
+                l_f[w][lane] = frec[jl];
                 if constexpr(GS)
-                    if(a.second_order) l_g[GS ? w : 0][GS ? slot : 0][lane] = ((const GradRec<T> *)a.grec)[jl];
+                    if(a.second_order) l_g[GS ? w : 0][lane] = ((const GradRec<T> *)a.grec)[jl];
             }
-            l_j[w][slot][lane] = jl;
+            l_j[w][lane] = jl;
             return valid ? (stream_ld(a.mf + o + lane) | stream_ld(a.mb + o + lane)) : 0ull;
         };
-        auto pair = [&](int slot, int kc) {
-            const int j = l_j[w][slot][kc];
-            const double4 pj4 = l_p[w][slot][kc];
+        auto pair = [&](int kc) {
+            const int j = l_j[w][kc];
+            const double4 pj4 = l_p[w][kc];
             double xj[D], d[D];
             xj[0] = pj4.x;
             if constexpr(D > 1) xj[1] = pj4.y;
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             T rij[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) rij[l] = (T)d[l];
-            const ForceRec<T> fj = l_f[w][slot][kc];
+            const ForceRec<T> fj = l_f[w][kc];
             const Kern kj(T(0), fj.hinv);  // the kernel object reads 1/h only
             const T cwi = ki.dw_coef(ki.q_of(r), r);
             const T cwj = kj.dw_coef(kj.q_of(r), r);
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     const T di = T(0.5) * (T(1) - ci * dt * rinv);
                     const T dj = T(0.5) * (T(1) - cj * dt * rinv);
                     const T dv = ve_i - ve_j;
-                    const GradRec<T> &grj = l_g[GS ? w : 0][GS ? slot : 0][GS ? kc : 0];
+                    const GradRec<T> &grj = l_g[GS ? w : 0][GS ? kc : 0];
                     T gvi[D], gvj[D];
 #pragma unroll
                     for(int c = 0; c < D; ++c) {
@@ -331,8 +331,8 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             }
         };
         // the cut-off r < max(h_i, h_j), r > 0 of the pair, tested ahead
-        auto keep = [&](int slot, int kc) -> bool {
-            const double4 pj4 = l_p[w][slot][kc];
+        auto keep = [&](int kc) -> bool {
+            const double4 pj4 = l_p[w][kc];
             double xj[D], d[D];
             xj[0] = pj4.x;
             if constexpr(D > 1) xj[1] = pj4.y;
@@ -346,8 +346,13 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                 return r2 < hmax * hmax && r2 != 0.0;
             }
         };
-        auto refine = [&](int slot, uint64_t m) { return refine_bits(slot, m, keep); };
-        tile_sweep(ntile, stage, refine, pair);
+        // refined in 3D only, see tiles.hpp
+        if constexpr(D == 3) {
+            auto refine = [&](uint64_t m) { return refine_bits(m, keep); };
+            tile_sweep(ntile, stage, refine, pair);
+        } else {
+            tile_sweep(ntile, stage, pair);
+        }
         if(valid) {
 #pragma unroll
             for(int k = 0; k < D; ++k) s.f[F_A + k][i] = acc[k].val();

@@ -117,7 +117,7 @@ template <int D, int K, class T>
 __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box, PreArgs a)
 {
     using Kern = typename KernelOf<D, K, T>::type;
-    __shared__ PosMass<T> l_r[PSW][2][WAVE];
+    __shared__ PosMass<T> l_r[PSW][WAVE];
     const int w = threadIdx.x / WAVE;
     const int lane = wave_lane();
     const int grp = xcd_block() * PSW + w;
@@ -134,22 +134,22 @@ __global__ void __launch_bounds__(BLOCK) pre_initial_kernel(State s, BoxDev box,
     Acc<T> rho;
     const size_t gbase = (size_t)grp * a.wcap;
     const int ntile = (a.gnc[grp] + WAVE - 1) / WAVE;
-    auto stage = [&](int tt, int slot) -> uint64_t {
+    auto stage = [&](int tt) -> uint64_t {
         const size_t o = gbase + (size_t)tt * WAVE;
         const int jl = stream_ld(a.cand + o + lane);
         if(jl >= 0) {
 #pragma unroll
-            for(int k = 0; k < D; ++k) l_r[w][slot][lane].x[k] = s.f[F_X + k][jl];
-            l_r[w][slot][lane].m = fld<T>(s, F_MASS, jl);
+            for(int k = 0; k < D; ++k) l_r[w][lane].x[k] = s.f[F_X + k][jl];
+            l_r[w][lane].m = fld<T>(s, F_MASS, jl);
         }
         return valid ? stream_ld(a.mf + o + lane) : 0ull;
     };
-    auto pair = [&](int slot, int kc) {
+    auto pair = [&](int kc) {
         double xj[D], d[D];
 #pragma unroll
-        for(int k = 0; k < D; ++k) xj[k] = l_r[w][slot][kc].x[k];
+        for(int k = 0; k < D; ++k) xj[k] = l_r[w][kc].x[k];
         const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
-        if(r < h) rho.add(l_r[w][slot][kc].m * ker.w(ker.q_of(r)));
+        if(r < h) rho.add(l_r[w][kc].m * ker.w(ker.q_of(r)));
     };
     tile_sweep(ntile, stage, pair);
     if(valid) s.f[F_DENS][i] = rho.val();
@@ -160,14 +160,14 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
 {
     using Kern = typename KernelOf<D, K, T>::type;
     constexpr bool NUMBER = TYPE == SPHGPU_DISPH;
-    __shared__ PreRec<T> l_r[PSW][2][WAVE];
-    __shared__ int l_j[PSW][2][WAVE];
+    __shared__ PreRec<T> l_r[PSW][WAVE];
+    __shared__ int l_j[PSW][WAVE];
     const int w = threadIdx.x / WAVE;
     // the Newton sweeps stage 32-byte position and mass records in the same
-    // LDS as the final sweep's records (2 x 64 of them per wave)
-    double4 *const l_pm = reinterpret_cast<double4 *>(&l_r[w][0][0]);
-    static_assert(sizeof(PreRec<T>) >= sizeof(double4) && (2 * WAVE * sizeof(PreRec<T>)) % alignof(double4) == 0,
-                  "position and mass records fit the final sweep's slots");
+    // LDS as the final sweep's records (64 of them per wave)
+    double4 *const l_pm = reinterpret_cast<double4 *>(&l_r[w][0]);
+    static_assert(sizeof(PreRec<T>) >= sizeof(double4) && (WAVE * sizeof(PreRec<T>)) % alignof(double4) == 0,
+                  "position and mass records fit the final sweep's slot");
     const int lane = wave_lane();
     // waves past the last group take part in the block's reduction only
     const int grp = xcd_block() * PSW + w;
@@ -202,14 +202,14 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
                 const T ht = (T)hn;
                 const Kern ker(ht);
                 Acc<T> rho, drho;
-                auto nstage = [&](int tt, int slot) -> uint64_t {
+                auto nstage = [&](int tt) -> uint64_t {
                     const size_t o = gbase + (size_t)tt * WAVE;
                     const int jl = stream_ld(a.cand + o + lane);
-                    if(jl >= 0) l_pm[slot * WAVE + lane] = a.pm[jl];
+                    if(jl >= 0) l_pm[lane] = a.pm[jl];
                     return done ? 0ull : stream_ld(a.mf + o + lane);
                 };
-                auto npair = [&](int slot, int kc) {
-                    const double4 pj = l_pm[slot * WAVE + kc];
+                auto npair = [&](int kc) {
+                    const double4 pj = l_pm[kc];
                     double xj[D], d[D];
                     xj[0] = pj.x;
                     if constexpr(D > 1) xj[1] = pj.y;
@@ -262,21 +262,21 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
         const bool need_dv = (TYPE != SPHGPU_GSPH && (a.balsara || a.tdav)) || (TYPE == SPHGPU_GSPH && a.second_order);
         const T ci_t = (T)ci, ui_t = (T)ui;
         int count = 0;
-        auto fstage = [&](int tt, int slot) -> uint64_t {
+        auto fstage = [&](int tt) -> uint64_t {
             const size_t o = gbase + (size_t)tt * WAVE;
             const int jl = stream_ld(a.cand + o + lane);
-            if(jl >= 0) l_r[w][slot][lane] = prec[jl];
-            l_j[w][slot][lane] = jl;
+            if(jl >= 0) l_r[w][lane] = prec[jl];
+            l_j[w][lane] = jl;
             return valid ? stream_ld(a.mf + o + lane) : 0ull;
         };
-        auto fpair = [&](int slot, int kc) {
-            const PreRec<T> pj = l_r[w][slot][kc];
+        auto fpair = [&](int kc) {
+            const PreRec<T> pj = l_r[w][kc];
             double xj[D], d[D];
 #pragma unroll
             for(int l = 0; l < D; ++l) xj[l] = pj.x[l];
             const T r = gather_r<T>(sep2<D>(box, xi, xj, d));
             if(!(r < ht)) return;
-            const int j = l_j[w][slot][kc];
+            const int j = l_j[w][kc];
             ++count;
             const T mj = pj.m;
             const T q = ker.q_of(r);
@@ -337,14 +337,20 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
             }
         };
         // the cut-off r < h of the final sums, tested ahead
-        auto fkeep = [&](int slot, int kc) -> bool {
+        auto fkeep = [&](int kc) -> bool {
             double xj[D], d[D];
 #pragma unroll
-            for(int l = 0; l < D; ++l) xj[l] = l_r[w][slot][kc].x[l];
+            for(int l = 0; l < D; ++l) xj[l] = l_r[w][kc].x[l];
             return gather_r<T>(sep2<D>(box, xi, xj, d)) < ht;
         };
-        auto frefine = [&](int slot, uint64_t m) { return refine_bits(slot, m, fkeep); };
-        tile_sweep(ntile, fstage, frefine, fpair);
+        // refined in 3D except for GSPH (measured slower on the 3D Sedov
+        // case), see tiles.hpp
+        if constexpr(D == 3 && TYPE != SPHGPU_GSPH) {
+            auto frefine = [&](uint64_t m) { return refine_bits(m, fkeep); };
+            tile_sweep(ntile, fstage, frefine, fpair);
+        } else {
+            tile_sweep(ntile, fstage, fpair);
+        }
         if(valid) {
             pairs = count;
             const double gm1 = a.gamma - 1.0;
