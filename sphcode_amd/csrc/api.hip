@@ -236,8 +236,14 @@ static int stage_pre(Ctx &c, bool walk = false)
         // and gains little from more waves, while the walk is bound by
         // instruction issue (measured per mixed Evrard step of 2^20
         // particles: 7.04 ms, against 7.16 ms with three blocks per CU and
-        // 7.35 ms with four).
-        const int bpc = walk ? 2 : 8;
+        // 7.35 ms with four).  Without gravity the best share depends on
+        // the dimension: the 2D KHI search (393,160 particles) takes
+        // 0.56 ms with two blocks per CU against 0.82 ms with eight (fewer
+        // waves keep their staging and tiles in L2), while the 3D Sedov
+        // step (2^24 particles) takes 65.8 ms with eight against 68.4 ms
+        // with four and 87.4 ms with two (its tree walks need the waves to
+        // hide their dependent loads).
+        const int bpc = (walk || c.D < 3) ? 2 : 8;
         if(walk && (rc = launch_gravity_walk(c))) return rc;
         walk = false;
         kmark(c, 0);

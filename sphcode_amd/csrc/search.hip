@@ -485,14 +485,14 @@ int search_gather(Ctx &c, bool initial)
     a.guess = 1;
     a.ratio = initial ? 1.0 : (c.p.iterative_sml ? 1.2 : 1.0);  // src/pre_interaction.cpp:31-35
     a.nngb = (double)c.p.neighbor_number;
-    return search_dispatch<false>(c, a);
+    return search_dispatch<false>(c, a, false, c.D < 3 ? 2 : 8);  // blocks per CU: see stage_pre
 }
 
 int search_symmetric(Ctx &c)
 {
     SearchArgs a = base_args(c);
     a.guess = 0;
-    return search_dispatch<true>(c, a);
+    return search_dispatch<true>(c, a, false, c.D < 3 ? 2 : 8);
 }
 
 // Slabs of candidate tiles for the c.ngroups groups of the last tree, and
