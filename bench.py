@@ -87,7 +87,13 @@ def cpu_baseline(params, fields, n: int, steps: int, warmup: int = 1) -> dict:
             "host": host_cpu(), "stage_s_per_step": stages}
 
 
-def timed_run(params, fields, steps: int, warmup: int, label: str):
+def timed_run(params, fields, steps: int, warmup: int, label: str, gravity_alone: int = 0):
+    """Times `steps` sphgpu_integrate calls after `warmup` untimed ones.
+    With gravity_alone > 0 (and gravity on), the gravity stage is then also
+    timed on its own: that many GravityForce stage calls (walk, pair pass
+    and combine in order on one stream, nothing beside them) on the state
+    the timed steps left, with their interaction counts; the engine is
+    discarded afterwards (each call adds gravity to acc again)."""
     from sphcode_amd import GPUEngine
     g = GPUEngine(params)
     g.upload(fields)
@@ -106,11 +112,30 @@ def timed_run(params, fields, steps: int, warmup: int, label: str):
     g.synchronize()
     elapsed = time.perf_counter() - t0
     stats, timers = g.stats(), g.timers()
+    alone = None
+    if gravity_alone > 0 and params.gravity_is_valid:
+        from sphcode_amd import costs
+        g.gravity_force()  # untimed: packs the records once
+        g.synchronize()
+        s0 = g.stats()
+        t0 = time.perf_counter()
+        for _ in range(gravity_alone):
+            g.gravity_force()
+        g.synchronize()
+        dt_alone = time.perf_counter() - t0
+        s1 = g.stats()
+        d = {k: s1[k] - s0[k] for k in ("gravity_pp", "gravity_pn", "gravity_opened")}
+        alone = {"calls": gravity_alone, "launch_ms": dt_alone / gravity_alone * 1e3,
+                 "flops_per_launch": costs.gravity_flops(d) / gravity_alone}
     g.close()
     n = len(fields["mass"])
     log(f"[bench] {label}: {steps} steps in {elapsed:.3f} s -> {n * steps / elapsed:.4g} particle-steps/s")
     log(f"[bench] {label}: stage ms/step: " + ", ".join(f"{k}={v / steps:.3f}" for k, v in timers.items()))
     log(f"[bench] {label}: stats: {stats}")
+    if alone is not None:
+        log(f"[bench] {label}: gravity stage alone {alone['launch_ms']:.3f} ms")
+        timers = dict(timers)
+        timers["_gravity_alone"] = alone
     return elapsed, stats, timers
 
 
@@ -321,7 +346,8 @@ def main() -> int:
             el64, _, _ = timed_run(p64, fields, args.steps, args.warmup, "fp64")
             fp64_rate = n * args.steps / el64
 
-    elapsed, stats, timers = timed_run(main_p, fields, args.steps, args.warmup, args.precision)
+    elapsed, stats, timers = timed_run(main_p, fields, args.steps, args.warmup, args.precision, gravity_alone=5)
+    grav_alone = timers.pop("_gravity_alone", None)
     value = n * args.steps / elapsed
     if args.precision == "fp64":
         fp64_rate = value
@@ -345,6 +371,12 @@ def main() -> int:
                                          max(64 * stats["gravity_wave_steps"], 1),
                 # particle pairs over the lane slots the pair pass evaluates
                 "pair_lane_utilisation": stats["gravity_pp"] / max(stats["gravity_pp_slots"], 1)}
+    if grav_alone is not None:
+        # the same two kernels (plus the combine) with nothing beside them:
+        # GravityForce stage calls after the timed steps
+        a_tf = grav_alone["flops_per_launch"] / (grav_alone["launch_ms"] * 1e-3) / 1e12
+        roofline["alone"] = {"launch_ms": grav_alone["launch_ms"], "flops_per_launch": grav_alone["flops_per_launch"],
+                             "achieved": a_tf, "frac": a_tf / peak}
     kernels = kernel_rooflines(n, stats, timers, args.steps, peak)
 
     adjoint = None
