@@ -231,6 +231,44 @@ def test_gravity_interaction_counts(gpu, case):
     gpu_e.close()
 
 
+def test_search_truncation_fallback(gpu):
+    """A homologous expansion v = kappa x over one fixed step with
+    kappa dt = 0.26: the density falls to half, so the smoothing length of
+    interior particles outgrows the search radius 1.2 h_guess that the
+    combined search used.  Newton then runs on the truncated gather set, as
+    the reference's does (src/pre_interaction.cpp:70), and the combined
+    search's candidate tiles no longer hold every force pair, so the step
+    falls back to the cells' new largest h and a symmetric search.  The
+    state after the step must be the oracle's."""
+    from sphcode_amd import GPUEngine, ics
+    p, f = ics.evrard(4096, leaf=32)
+    p.gravity_is_valid = 0
+    dt = 0.01
+    f = dict(f)
+    f["vel"] = (0.26 / dt) * np.asarray(f["pos"], dtype=np.float64)
+    ref = OracleEngine(p)
+    gpu_e = GPUEngine(p)
+    for e in (ref, gpu_e):
+        e.upload(f)
+        e.initialize()
+        e.integrate(dt)
+    sa, sb = ref.stats(), gpu_e.stats()
+    print(f"[trunc] oracle truncations={sa['search_truncations']} gpu truncations={sb['search_truncations']}", flush=True)
+    assert sb["search_truncations"] > 0
+    assert sb["search_truncations"] == sa["search_truncations"]
+    a, b = ref.download(), gpu_e.download()
+    for k in ("sml", "dens", "pres"):
+        assert _rel(b[k], a[k]) <= 1e-9, f"{k} rel err {_rel(b[k], a[k]):.3e}"
+    # DISPH grad-h on the scale it enters f_ij = 1 - gradh_i / (m_j u_j)
+    mu = np.median(a["mass"] * a["ene"])
+    assert np.max(np.abs(b["gradh"] - a["gradh"])) <= 1e-9 * mu
+    for k in ("pos", "vel"):
+        assert _rms_err(b[k], a[k]) <= 1e-10, k
+    assert _rms_err(b["acc"], a["acc"]) <= 1e-8, f"acc err {_rms_err(b['acc'], a['acc']):.3e}"
+    assert _rms_err(b["dene"], a["dene"]) <= 1e-8
+    gpu_e.close()
+
+
 def test_midrun_state_parity(gpu):
     """A state with nonzero velocities and evolved alpha, dt > 0 (TDAV)."""
     from sphcode_amd import GPUEngine
