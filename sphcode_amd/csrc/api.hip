@@ -264,6 +264,16 @@ static int stage_pre(Ctx &c, bool walk = false)
     if(combined) {
         // the combined lists hold every force pair unless some h outgrew its
         // search radius (src/pre_interaction.cpp:70 truncates the gather set)
+        if(c.speculate && !c.need_lists && !c.record) {
+            // sphgpu_integrate: the force pass starts on the combined lists
+            // at once and stage_force checks the counters once it is queued
+            // (a truncation, which needs a new search, is rare)
+            SPHGPU_HIP(hipMemcpyAsync(c.counters_spec, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+            SPHGPU_HIP(hipEventRecord(c.ev_flags, c.stream));
+            c.flags_pending = true;
+            c.lists_symmetric = true;
+            return SPHGPU_OK;
+        }
         SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
         SPHGPU_HIP(hipStreamSynchronize(c.stream));
         const int64_t tr = c.counters_host[C_TRUNC];
@@ -286,6 +296,27 @@ static int stage_force(Ctx &c)
     kmark(c, 4);
     if((rc = force_compute(c))) return rc;
     kmark(c, 5);
+    if(c.flags_pending) {
+        // the speculative force pass is queued: check the combined lists now
+        c.flags_pending = false;
+        SPHGPU_HIP(hipEventSynchronize(c.ev_flags));
+        const int64_t tr = c.counters_spec[C_TRUNC];
+        const bool ok = tr == c.trunc_seen && c.counters_spec[C_OVERFLOW] == 0;
+        c.trunc_seen = tr;
+        if(!ok) {
+            // some h outgrew its search radius: the cells' largest h, a
+            // symmetric search, and the force pass again (it assigns acc and
+            // du/dt in full; the gravity pair pass started by the first one
+            // is not started again)
+            int (*hook)(Ctx &) = c.after_pack;
+            c.after_pack = nullptr;
+            if((rc = tree_set_kernel(c)) || (rc = search_symmetric(c)) || (rc = force_compute(c))) {
+                c.after_pack = hook;
+                return rc;
+            }
+            c.after_pack = hook;
+        }
+    }
     if(c.record) c.have_force = true;
     return SPHGPU_OK;
 }
@@ -381,6 +412,8 @@ int sphgpu_create(const sphgpu_params *params, void **out)
        (e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_flags, hipEventDisableTiming)) != hipSuccess ||
+       (e = hipHostMalloc(&c->counters_spec, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
        (e = hipEventCreate(&c->ev_g[0])) != hipSuccess || (e = hipEventCreate(&c->ev_g[1])) != hipSuccess ||
        (e = hipEventCreate(&c->ev_g[2])) != hipSuccess || (e = hipEventCreate(&c->ev_g[3])) != hipSuccess ||
        (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
@@ -426,11 +459,12 @@ int sphgpu_destroy(void *ptr)
     if(c->counters) (void)hipFree(c->counters);
     if(c->work) (void)hipFree(c->work);
     if(c->counters_host) (void)hipHostFree(c->counters_host);
+    if(c->counters_spec) (void)hipHostFree(c->counters_spec);
     if(c->scal_host) (void)hipHostFree(c->scal_host);
     for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     if(c->gstream) (void)hipStreamDestroy(c->gstream);
-    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_g[0], c->ev_g[1], c->ev_g[2], c->ev_g[3]})
+    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_flags, c->ev_g[0], c->ev_g[1], c->ev_g[2], c->ev_g[3]})
         if(e) (void)hipEventDestroy(e);
     delete c;
     return SPHGPU_OK;
@@ -652,7 +686,10 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     // acc once both streams are done.
     const bool grav = c.p.gravity_is_valid;
     mark(3);
+    c.speculate = true;
     if((rc = stage_pre(c, grav))) {
+        c.speculate = false;
+        c.flags_pending = false;
         // later work on the main stream must not overtake the walk
         if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
         return rc;
@@ -662,6 +699,8 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     c.after_pack = grav ? launch_gravity_leaves : nullptr;
     rc = stage_force(c);
     c.after_pack = nullptr;
+    c.speculate = false;
+    c.flags_pending = false;
     if(rc) {
         if(grav) (void)hipStreamWaitEvent(c.stream, c.ev_grav, 0);
         return rc;

@@ -183,6 +183,13 @@ struct Ctx {
     // gravity pass alongside the force pass
     hipStream_t gstream = nullptr;
     hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[4] = {nullptr, nullptr, nullptr, nullptr};
+    // sphgpu_integrate's check of the combined lists, read without stalling
+    // the stream: the counters are copied after the pre-interaction (event
+    // ev_flags) and read once the force pass is queued
+    hipEvent_t ev_flags = nullptr;
+    bool speculate = false;      // sphgpu_integrate: assume symmetric lists, check after
+    bool flags_pending = false;  // a copy behind ev_flags awaits its check
+    int64_t *counters_spec = nullptr;  // pinned host copy for that check
     // called by the force pass between its record packing and its pair
     // kernel (sphgpu_integrate starts the gravity pair pass there, so that
     // the packing is not left waiting behind it); its status is kept here
