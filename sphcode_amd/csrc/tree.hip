@@ -365,13 +365,8 @@ __global__ void __launch_bounds__(BLOCK) level_scatter_kernel(TreeDev t, int *lv
 // particles, an internal node its children (one level deeper, done by the
 // previous launch), in child order.  The root's moments stay zero, as the
 // reference never accumulates them (include/bhtree.hpp:46-53).
-template <int D>
-__global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t, const int *nodes, int count, double4 *msum)
+template <int D> __device__ __forceinline__ void moments_node(const State &s, const TreeDev &t, int p, int total, double4 *msum)
 {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if(k >= count) return;
-    const int p = nodes[k];
-    const int total = *t.num_nodes;
     const int4 r = t.rng[p];
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     if(r.w & 256) {
@@ -395,6 +390,36 @@ __global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t
     msum[p] = acc;
     if(p == 0) t.mom[0] = make_double4(0.0, 0.0, 0.0, 0.0);
     else t.mom[p] = make_double4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
+}
+
+template <int D>
+__global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t, const int *nodes, int count, double4 *msum)
+{
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if(k >= count) return;
+    moments_node<D>(s, t, nodes[k], *t.num_nodes, msum);
+}
+
+// The upper levels, which hold few nodes, in one launch of one workgroup:
+// level after level bottom-up, a barrier between levels orders the
+// children's results before their parents read them (one workgroup runs on
+// one CU, whose waves see each other's stores once past the barrier).
+constexpr int TOP_THREADS = 1024;
+constexpr int TOP_MAXNODES = 4096;  // a level is one of the upper ones up to this many nodes
+struct TopLevels {
+    int nlev;         // levels 0 .. nlev-1
+    int count[16];
+    int offset[16];
+};
+
+template <int D>
+__global__ void __launch_bounds__(TOP_THREADS) moments_top_kernel(State s, TreeDev t, const int *nodes, TopLevels tl, double4 *msum)
+{
+    const int total = *t.num_nodes;
+    for(int L = tl.nlev - 1; L >= 0; --L) {
+        for(int k = threadIdx.x; k < tl.count[L]; k += TOP_THREADS) moments_node<D>(s, t, nodes[tl.offset[L] + k], total, msum);
+        __syncthreads();
+    }
 }
 
 // Search groups: compact sets of at most 64 key-consecutive particles.
@@ -464,12 +489,8 @@ __global__ void __launch_bounds__(BLOCK) groups_kernel(TreeDev t, int2 *groups, 
 
 // Largest smoothing length per subtree (BHNode::set_kernel,
 // src/bhtree.cpp:206-232), one level per launch as the moments.
-__global__ void __launch_bounds__(BLOCK) kernel_size_level_kernel(State s, TreeDev t, const int *nodes, int count)
+__device__ __forceinline__ void kernel_size_node(const State &s, const TreeDev &t, int p, int total)
 {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if(k >= count) return;
-    const int p = nodes[k];
-    const int total = *t.num_nodes;
     const int4 r = t.rng[p];
     double h = 0.0;
     if(r.w & 256) {
@@ -485,6 +506,22 @@ __global__ void __launch_bounds__(BLOCK) kernel_size_level_kernel(State s, TreeD
         }
     }
     t.ks[p] = h;
+}
+
+__global__ void __launch_bounds__(BLOCK) kernel_size_level_kernel(State s, TreeDev t, const int *nodes, int count)
+{
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if(k >= count) return;
+    kernel_size_node(s, t, nodes[k], *t.num_nodes);
+}
+
+__global__ void __launch_bounds__(TOP_THREADS) kernel_size_top_kernel(State s, TreeDev t, const int *nodes, TopLevels tl)
+{
+    const int total = *t.num_nodes;
+    for(int L = tl.nlev - 1; L >= 0; --L) {
+        for(int k = threadIdx.x; k < tl.count[L]; k += TOP_THREADS) kernel_size_node(s, t, nodes[tl.offset[L] + k], total);
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -535,6 +572,21 @@ int tree_alloc(Ctx &c)
     c.scan_tmp_bytes = bytes;
     SPHGPU_HIP(hipMalloc(&c.scan_tmp, bytes));
     return SPHGPU_OK;
+}
+
+// The upper levels that one workgroup reduces: levels 0 .. nlev-1, each with
+// at most TOP_MAXNODES nodes (level counts from the last tree build).
+static TopLevels top_levels(const Ctx &c)
+{
+    TopLevels tl{};
+    int L = 0;
+    while(L < 16 && L < NLEVEL && c.lvl_host[L] > 0 && c.lvl_host[L] <= TOP_MAXNODES) {
+        tl.count[L] = c.lvl_host[L];
+        tl.offset[L] = c.lvl_host[NLEVEL + L];
+        ++L;
+    }
+    tl.nlev = L;
+    return tl;
 }
 
 template <int D> static int tree_build_d(Ctx &c)
@@ -591,12 +643,15 @@ template <int D> static int tree_build_d(Ctx &c)
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     c.ngroups = c.lvl_host[2 * NLEVEL];
     if(c.ngroups < 1 || c.ngroups > c.group_cap) return set_error(SPHGPU_E_TREE, "search groups out of range");
-    for(int L = NLEVEL - 1; L >= 0; --L) {
+    const TopLevels tl = top_levels(c);
+    for(int L = NLEVEL - 1; L >= tl.nlev; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
         hipLaunchKernelGGL(moments_level_kernel<D>, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
                            c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
     }
+    if(tl.nlev > 0)
+        hipLaunchKernelGGL(moments_top_kernel<D>, dim3(1), dim3(TOP_THREADS), 0, c.stream, c.s, c.t, c.lvl_nodes, tl, c.msum);
     SPHGPU_HIP(hipGetLastError());
     return search_alloc(c);
 }
@@ -613,12 +668,15 @@ int tree_build(Ctx &c)
 int tree_set_kernel(Ctx &c)
 {
     // level lists and counts from the last tree build
-    for(int L = NLEVEL - 1; L >= 0; --L) {
+    const TopLevels tl = top_levels(c);
+    for(int L = NLEVEL - 1; L >= tl.nlev; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
         hipLaunchKernelGGL(kernel_size_level_kernel, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
                            c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt);
     }
+    if(tl.nlev > 0)
+        hipLaunchKernelGGL(kernel_size_top_kernel, dim3(1), dim3(TOP_THREADS), 0, c.stream, c.s, c.t, c.lvl_nodes, tl);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
