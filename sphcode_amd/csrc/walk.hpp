@@ -81,7 +81,16 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
     const char *gnb = reinterpret_cast<const char *>(gn);
     do {
         ++wc.visits;
-        const GravNode g = load_ro(reinterpret_cast<const GravNode *>(gnb + ((uint32_t)node << 6)));
+        // the whole 64-byte record in one scalar load (the compiler would
+        // otherwise split it by the fields used into five); the empty asm
+        // statement only keeps all sixteen words live
+        GravNode g;
+        {
+            typedef uint32_t u16v __attribute__((ext_vector_type(16)));
+            const u16v raw = *(const __attribute__((address_space(4))) u16v *)(gnb + ((uint32_t)node << 6));
+            asm volatile("" ::"s"(raw));
+            __builtin_memcpy(&g, &raw, sizeof(GravNode));
+        }
         const int skip = g.skip;
         const uint64_t om = v.node(g, active);
         const int nopen = __popcll(om);
