@@ -162,50 +162,73 @@ struct TreeShape {
 // First level at which particle b begins a cell.
 __device__ __forceinline__ int first_level(const uint8_t *delta, int b) { return b == 0 ? 2 : (int)delta[b] + 2; }
 
-// Size of the level-L cell that begins at b, or leaf+1 if it is larger
-// than leaf.  A later particle j begins a new level-L cell iff
-// delta[j] < L-1.
-__device__ __forceinline__ int cell_count_bounded(const uint8_t *delta, const TreeShape &ts, int b, int L)
+// Level-L cells are the runs of sorted keys that share their top L-1 levels
+// (the prefix cell_end compares).  The two helpers below find a run's ends
+// within leaf positions of b by binary search over the keys, so every lane
+// takes the same few steps (a scan of the key-difference bytes took up to
+// 2 leaf steps, different for each lane).
+__device__ __forceinline__ uint64_t cell_prefix(uint64_t k, int shift) { return shift >= 64 ? 0 : k >> shift; }
+
+// Size of the level-L cell that begins at b, or leaf+1 if it is larger.
+__device__ __forceinline__ int cell_count_bounded(const uint64_t *keys, const TreeShape &ts, int b, int L)
 {
-    for(int j = b + 1; j <= b + ts.leaf; ++j) {
-        if(j >= ts.n) return ts.n - b;
-        if((int)delta[j] < L - 1) return j - b;
+    const int shift = (ts.G - (L - 1)) * ts.D;
+    const uint64_t pre = cell_prefix(keys[b], shift);
+    // last member of the run in [b, b + leaf]
+    int lo = b, hi = min(ts.n - 1, b + ts.leaf);
+    while(lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if(cell_prefix(keys[mid], shift) > pre) hi = mid - 1;
+        else lo = mid;
     }
-    return ts.leaf + 1;
+    return lo - b + 1;  // leaf + 1 when the run reaches b + leaf
 }
 
-// Does the level-P cell containing b-1 and b hold more than leaf particles?
-__device__ __forceinline__ bool parent_split(const uint8_t *delta, const TreeShape &ts, int b, int P)
+// Does the level-P cell containing b hold more than leaf particles?  Its
+// members within leaf positions on either side of b number more than leaf
+// exactly when the whole cell does.
+__device__ __forceinline__ bool parent_split(const uint64_t *keys, const TreeShape &ts, int b, int P)
 {
-    int cnt = 1;
-    int j = b;
-    while(cnt <= ts.leaf && j > 0 && (int)delta[j] >= P - 1) { --j; ++cnt; }
-    int k = b + 1;
-    while(cnt <= ts.leaf && k < ts.n && (int)delta[k] >= P - 1) { ++k; ++cnt; }
-    return cnt > ts.leaf;
+    const int shift = (ts.G - (P - 1)) * ts.D;
+    const uint64_t pre = cell_prefix(keys[b], shift);
+    int lo = max(0, b - ts.leaf), hi = b;  // first member in [b - leaf, b]
+    while(lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if(cell_prefix(keys[mid], shift) < pre) lo = mid + 1;
+        else hi = mid;
+    }
+    const int first = lo;
+    lo = b;
+    hi = min(ts.n - 1, b + ts.leaf);  // last member in [b, b + leaf]
+    while(lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if(cell_prefix(keys[mid], shift) > pre) hi = mid - 1;
+        else lo = mid;
+    }
+    return lo - first + 1 > ts.leaf;
 }
 
 // Number of nodes that begin at particle b (all at consecutive levels).
-__device__ int nodes_at(const uint8_t *delta, const TreeShape &ts, int b, int &lvl0)
+__device__ int nodes_at(const uint64_t *keys, const uint8_t *delta, const TreeShape &ts, int b, int &lvl0)
 {
     if(b > 0 && (int)delta[b] >= ts.G) return 0;  // same deepest cell as b-1
     lvl0 = first_level(delta, b);
-    if(lvl0 > 2 && !parent_split(delta, ts, b, lvl0 - 1)) return 0;
+    if(lvl0 > 2 && !parent_split(keys, ts, b, lvl0 - 1)) return 0;
     int k = 0;
     for(int L = lvl0;; ++L) {
         ++k;
-        const int cnt = cell_count_bounded(delta, ts, b, L);
+        const int cnt = cell_count_bounded(keys, ts, b, L);
         if(!(cnt > ts.leaf && L <= ts.max_level)) break;
     }
     return k;
 }
 
-__global__ void __launch_bounds__(BLOCK) count_nodes_kernel(const uint8_t *delta, TreeShape ts, int *nn)
+__global__ void __launch_bounds__(BLOCK) count_nodes_kernel(const uint64_t *keys, const uint8_t *delta, TreeShape ts, int *nn)
 {
     const int b = blockIdx.x * BLOCK + threadIdx.x;
     if(b >= ts.n) return;
     int lvl0 = 0;
-    nn[b] = nodes_at(delta, ts, b, lvl0);
+    nn[b] = nodes_at(keys, delta, ts, b, lvl0);
 }
 
 // End (exclusive) of the level-L cell that begins at b: first particle
@@ -242,7 +265,7 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
     const int cnt = nn[b];
     if(cnt == 0) return;
     int lvl0 = 0;
-    nodes_at(delta, ts, b, lvl0);
+    nodes_at(keys, delta, ts, b, lvl0);
     const int base = 1 + noff[b];
     const uint64_t key = keys[b];
     const uint64_t mask = (1ull << D) - 1ull;
@@ -269,7 +292,7 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
         const int node = base + j;
         const bool leaf = j == cnt - 1;
         int end;
-        if(leaf && L <= ts.max_level) end = b + cell_count_bounded(delta, ts, b, L);
+        if(leaf && L <= ts.max_level) end = b + cell_count_bounded(keys, ts, b, L);
         else end = cell_end(keys, ts, b, L);
         const int skip = end < ts.n ? 1 + noff[end] : total;
         if(node < node_cap) {
@@ -405,9 +428,10 @@ __global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t
 // children's results before their parents read them (one workgroup runs on
 // one CU, whose waves see each other's stores once past the barrier).
 constexpr int TOP_THREADS = 1024;
-constexpr int TOP_MAXNODES = 4096;  // a level is one of the upper ones up to this many nodes
+constexpr int TOP_MAXNODES = 512;  // a level is one of the upper ones up to this many nodes
 struct TopLevels {
-    int nlev;         // levels 0 .. nlev-1
+    int first;        // levels first .. first + nlev - 1 (entry k is level first + k)
+    int nlev;
     int count[16];
     int offset[16];
 };
@@ -580,12 +604,15 @@ static TopLevels top_levels(const Ctx &c)
 {
     TopLevels tl{};
     int L = 0;
-    while(L < 16 && L < NLEVEL && c.lvl_host[L] > 0 && c.lvl_host[L] <= TOP_MAXNODES) {
-        tl.count[L] = c.lvl_host[L];
-        tl.offset[L] = c.lvl_host[NLEVEL + L];
-        ++L;
+    while(L < NLEVEL && c.lvl_host[L] == 0) ++L;  // levels start at the root's
+    tl.first = L;
+    int k = 0;
+    while(k < 16 && L + k < NLEVEL && c.lvl_host[L + k] > 0 && c.lvl_host[L + k] <= TOP_MAXNODES) {
+        tl.count[k] = c.lvl_host[L + k];
+        tl.offset[k] = c.lvl_host[NLEVEL + L + k];
+        ++k;
     }
-    tl.nlev = L;
+    tl.nlev = k;
     return tl;
 }
 
@@ -621,7 +648,7 @@ template <int D> static int tree_build_d(Ctx &c)
 
     const TreeShape ts{n, G, D, c.p.tree_leaf_particle_num, c.p.tree_max_level};
     hipLaunchKernelGGL(delta_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, n, G, D, c.delta);
-    hipLaunchKernelGGL(count_nodes_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.delta, ts, c.nn);
+    hipLaunchKernelGGL(count_nodes_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, c.delta, ts, c.nn);
     bytes = c.scan_tmp_bytes;
     SPHGPU_HIP(rocprim::exclusive_scan(c.scan_tmp, bytes, c.nn, c.noff, 0, (size_t)n, rocprim::plus<int>(), c.stream));
     hipLaunchKernelGGL(emit_nodes_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, c.delta, ts, c.nn, c.noff,
@@ -644,7 +671,7 @@ template <int D> static int tree_build_d(Ctx &c)
     c.ngroups = c.lvl_host[2 * NLEVEL];
     if(c.ngroups < 1 || c.ngroups > c.group_cap) return set_error(SPHGPU_E_TREE, "search groups out of range");
     const TopLevels tl = top_levels(c);
-    for(int L = NLEVEL - 1; L >= tl.nlev; --L) {
+    for(int L = NLEVEL - 1; L >= tl.first + tl.nlev; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
         hipLaunchKernelGGL(moments_level_kernel<D>, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
@@ -669,7 +696,7 @@ int tree_set_kernel(Ctx &c)
 {
     // level lists and counts from the last tree build
     const TopLevels tl = top_levels(c);
-    for(int L = NLEVEL - 1; L >= tl.nlev; --L) {
+    for(int L = NLEVEL - 1; L >= tl.first + tl.nlev; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
         hipLaunchKernelGGL(kernel_size_level_kernel, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
