@@ -677,7 +677,13 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     mark(1);
     if((rc = step_predict(c))) return rc;
     mark(2);
-    if((rc = stage_tree(c))) return rc;
+    // the combined pre-interaction (not the first, initial-smoothing one)
+    // rewrites h, 1/h, P, grad-h and the neighbour counts, and the force and
+    // gravity passes acc, du/dt and phi, before any kernel reads them
+    c.reorder_light = !c.first_pre;
+    rc = stage_tree(c);
+    c.reorder_light = false;
+    if(rc) return rc;
     // Gravity runs on the second stream in two parts: the tree walk (vector-
     // ALU bound; needs the tree and positions only), started once the tree
     // is built, beside the neighbour search (which waits on dependent loads)

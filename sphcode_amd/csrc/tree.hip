@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *pe
     const int j = perm[i];
     for(int f = 0; f < nfields; ++f) r.dst[f][i] = r.src[f][j];
     r.idst[0][i] = r.isrc[0][j];
-    r.idst[1][i] = r.isrc[1][j];
+    if(r.isrc[1]) r.idst[1][i] = r.isrc[1][j];
 }
 
 // ----------------------------------------------------------------- delta
@@ -616,6 +616,21 @@ static TopLevels top_levels(const Ctx &c)
     return tl;
 }
 
+// Fields that a sphgpu_integrate step writes after its tree build before
+// any kernel reads them: acc and du/dt (force pass), pressure, h and 1/h
+// (pre-interaction; h is first the search radius the guess writes),
+// grad-h (SSPH, DISPH), the Balsara switch where the pre-interaction sets it,
+// and phi (gravity walk).
+static bool rewritten_in_step(const Ctx &c, int f)
+{
+    if(f >= F_A && f < F_A + 3) return true;
+    if(f == F_DENE || f == F_PRES || f == F_SML || f == F_HINV) return true;
+    if(f == F_GRADH) return c.p.sph_type != SPHGPU_GSPH;
+    if(f == F_BAL) return c.p.av_use_balsara_switch && c.D != 1 && c.p.sph_type != SPHGPU_GSPH;
+    if(f == F_PHI) return c.p.gravity_is_valid != 0;
+    return false;
+}
+
 template <int D> static int tree_build_d(Ctx &c)
 {
     const int n = (int)c.n;
@@ -629,17 +644,19 @@ template <int D> static int tree_build_d(Ctx &c)
     hipLaunchKernelGGL(keys_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.s, n, c.scal, G, c.keys, c.idx);
     size_t bytes = c.sort_tmp_bytes;
     SPHGPU_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, c.keys, c.keys_sorted, c.idx, c.perm, (unsigned)n, 0, (unsigned)(G * D), c.stream));
-    // gather the state into key order
+    // gather the state into key order (in sphgpu_integrate, without the
+    // fields that the step rewrites before anything reads them)
     Reorder r{};
     int nf = 0;
     for(int f = 0; f < NF; ++f) {
         if(!c.s.f[f]) continue;
+        if(c.reorder_light && rewritten_in_step(c, f)) continue;
         r.src[nf] = c.s.f[f];
         r.dst[nf] = c.alt.f[f];
         ++nf;
     }
     r.isrc[0] = c.s.id; r.idst[0] = c.alt.id;
-    r.isrc[1] = c.s.nb; r.idst[1] = c.alt.nb;
+    r.isrc[1] = c.reorder_light ? nullptr : c.s.nb; r.idst[1] = c.alt.nb;
     hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, r, c.perm, n, nf);
     // swap master buffers; FP32 mirrors stay (they are rewritten before use)
     for(int f = 0; f < NF; ++f) std::swap(c.s.f[f], c.alt.f[f]);
