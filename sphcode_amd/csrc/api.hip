@@ -13,6 +13,12 @@ namespace sphgpu {
 
 static thread_local std::string g_error;
 
+// Events that only order the two streams or time spans need the data before
+// them visible to this device, not to the host.
+#ifndef EV_SCOPE
+#define EV_SCOPE hipEventReleaseToDevice
+#endif
+
 int set_error(int code, const std::string &msg)
 {
     g_error = msg;
@@ -410,12 +416,14 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     hipError_t e;
     if((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
        (e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming | EV_SCOPE)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming | EV_SCOPE)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_flags, hipEventDisableTiming)) != hipSuccess ||
        (e = hipHostMalloc(&c->counters_spec, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
-       (e = hipEventCreate(&c->ev_g[0])) != hipSuccess || (e = hipEventCreate(&c->ev_g[1])) != hipSuccess ||
-       (e = hipEventCreate(&c->ev_g[2])) != hipSuccess || (e = hipEventCreate(&c->ev_g[3])) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_g[0], EV_SCOPE)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_g[1], EV_SCOPE)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_g[2], EV_SCOPE)) != hipSuccess ||
+       (e = hipEventCreateWithFlags(&c->ev_g[3], EV_SCOPE)) != hipSuccess ||
        (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
        (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
@@ -424,7 +432,7 @@ int sphgpu_create(const sphgpu_params *params, void **out)
         delete c;
         return hip_fail(e, "sphgpu_create");
     }
-    for(int k = 0; k < 16; ++k) (void)hipEventCreate(&c->ev[k]);
+    for(int k = 0; k < 16; ++k) (void)hipEventCreateWithFlags(&c->ev[k], EV_SCOPE);
     hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c->stream, c->counters, NCOUNTERS);
     hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->scal, (int64_t)NSCALARS, 0.0);
     if(p.periodic_is_valid) {

@@ -34,6 +34,8 @@
 // pre-interaction and the force stage.
 #include "ctx.hpp"
 
+#include <type_traits>
+
 namespace sphgpu {
 
 struct SearchArgs {
@@ -89,8 +91,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     __shared__ int l_begin[SW][LEAFCAP];
     __shared__ int l_pre[SW][LEAFCAP + 1];
     __shared__ double l_ks[SW][LEAFCAP];
-    __shared__ double c_x[SW][3][WAVE];
-    __shared__ double c_lim[SW][WAVE];  // IJ: the candidate's radius squared
+    // the tile's candidates: position and (IJ) radius squared
+    __shared__ double4 c_p[SW][WAVE];
 
     const int w = uniform(threadIdx.x / WAVE);
     const int lane = wave_lane();
@@ -323,12 +325,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     for(int tt = 0; tt < T; ++tt) {
         const int m = (nc - tt + T - 1) / T;  // candidates of this tile (a prefix of its lanes)
         const int j = p_j;
-        if(j >= 0) {
-            c_x[w][0][lane] = p_p.x;
-            if constexpr(D > 1) c_x[w][1][lane] = p_p.y;
-            if constexpr(D > 2) c_x[w][2][lane] = p_p.z;
-            if constexpr(IJ) c_lim[w][lane] = p_p.w;
-        }
+        if(j >= 0) c_p[w][lane] = p_p;
         stream_st(a.cand + gbase + (size_t)tt * WAVE + lane, j);
         // next tile's records
         const int cn = tt + 1 + lane * T;
@@ -341,39 +338,51 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         // every lane tests every candidate of the tile (uniform loop,
         // broadcast LDS reads) into a hit mask, and a mask of the hits
         // inside its own radius (the gather entries of a split search;
-        // every hit otherwise)
+        // every hit otherwise).  Two candidates per pass, their records
+        // read together (one with periodic boundaries); the slots past the
+        // tile's m candidates hold stale records, whose bits are cleared.
+        // (Four per pass spilled registers at this kernel's budget and was
+        // no faster.)
         uint64_t mine = 0, mfront = 0;
         if(valid) {
-            uint32_t mlo = 0, mhi = 0, flo = 0, fhi = 0;
-            auto test = [&](int cc, uint32_t &hb, uint32_t &fb) {
-                double xj[D], d[D];
+            auto sweep = [&](auto periodic) {
+#pragma clang fp contract(off)
+                constexpr bool PER = decltype(periodic)::value;
+                constexpr int U = PER ? 1 : 2;
+                uint64_t hm = 0, fm = 0;
+                for(int cc = 0; cc < m; cc += U) {
+                    double4 q[U];
 #pragma unroll
-                for(int k = 0; k < D; ++k) xj[k] = c_x[w][k][cc];
-                const double r2 = sep2<D>(box, xi, xj, d);
-                // r2 < max(h_i^2, kl^2) as two tests; all three are
-                // non-negative, so their bit patterns order as integers
-                const int64_t rb = __double_as_longlong(r2);
-                const bool f = rb < h2_bits;
-                bool hh = f;
-                if constexpr(IJ) hh = f | (rb < __double_as_longlong(c_lim[w][cc]));
-                hb = (uint32_t)hh;
-                fb = (uint32_t)f;
+                    for(int u = 0; u < U; ++u) q[u] = c_p[w][cc + u];
+                    uint32_t hn = 0, fn = 0;
+#pragma unroll
+                    for(int u = 0; u < U; ++u) {
+                        double d[D];
+                        d[0] = xi[0] - q[u].x;
+                        if constexpr(D > 1) d[1] = xi[1] - q[u].y;
+                        if constexpr(D > 2) d[2] = xi[2] - q[u].z;
+                        if constexpr(PER) min_image<D>(box, d);
+                        double r2 = d[0] * d[0];
+#pragma unroll
+                        for(int k = 1; k < D; ++k) r2 = r2 + d[k] * d[k];
+                        // r2 < max(h_i^2, kl^2) as two tests; all three are
+                        // non-negative, so their bit patterns order as integers
+                        const int64_t rb = __double_as_longlong(r2);
+                        const bool f = rb < h2_bits;
+                        bool hh = f;
+                        if constexpr(IJ) hh = f | (rb < __double_as_longlong(q[u].w));
+                        hn |= (uint32_t)hh << u;
+                        fn |= (uint32_t)f << u;
+                    }
+                    hm |= (uint64_t)hn << cc;
+                    fm |= (uint64_t)fn << cc;
+                }
+                const uint64_t live = m >= WAVE ? ~0ull : (1ull << m) - 1ull;
+                mine = hm & live;
+                mfront = a.split ? fm & live : mine;
             };
-            const int m1 = m < 32 ? m : 32;
-            for(int cc = 0; cc < m1; ++cc) {
-                uint32_t hb, fb;
-                test(cc, hb, fb);
-                mlo |= hb << cc;
-                flo |= fb << cc;
-            }
-            for(int cc = 32; cc < m; ++cc) {
-                uint32_t hb, fb;
-                test(cc, hb, fb);
-                mhi |= hb << (cc - 32);
-                fhi |= fb << (cc - 32);
-            }
-            mine = ((uint64_t)mhi << 32) | mlo;
-            mfront = a.split ? ((uint64_t)fhi << 32) | flo : mine;
+            if(box.on) sweep(std::true_type{});
+            else sweep(std::false_type{});
         }
         const size_t mo = gbase + (size_t)tt * WAVE + lane;
         stream_st(a.mf + mo, mfront);
