@@ -86,7 +86,7 @@ static void free_buffers(Ctx &c)
     fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
-    fr(c.parent); fr(c.nchild); fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
+    fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
     fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.leafhit); fr(c.leafcnt); fr(c.posrec); fr(c.pair_rec); fr(c.grad_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;

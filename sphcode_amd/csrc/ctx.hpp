@@ -218,7 +218,6 @@ struct Ctx {
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     TreeDev t{};
-    int *parent = nullptr, *nchild = nullptr;
     int *lvl_nodes = nullptr;  // node ids grouped by tree level
     int *lvl = nullptr;        // per-level node counts, offsets, fill (device)
     int *lvl_host = nullptr;   // per-level counts and offsets (pinned host)

@@ -19,8 +19,9 @@
 // b begins a level-L cell whose parent holds more than leafParticleNumber
 // particles.  Each particle emits the nodes that start at it, so an
 // exclusive scan of those counts is the pre-order numbering.  Mass moments
-// and the largest h per subtree are accumulated bottom-up with one
-// arrival counter per node.  The root keeps zero mass and centre of mass,
+// and the largest h per subtree are reduced bottom-up, one tree level per
+// launch (the upper levels, which hold few nodes, in one single-workgroup
+// launch).  The root keeps zero mass and centre of mass,
 // as the reference never accumulates them (include/bhtree.hpp:46-53).
 #include <cstring>
 
@@ -144,23 +145,37 @@ __global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *pe
 // ----------------------------------------------------------------- delta
 // delta[i] = number of equal leading slot groups of keys i-1 and i: the two
 // particles share the cells of levels 1 .. delta+1.
-__global__ void __launch_bounds__(BLOCK) delta_kernel(const uint64_t *k, int n, int G, int D, uint8_t *delta)
+__device__ __forceinline__ int key_delta(const uint64_t *k, int i, int G, int D)
 {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if(i >= n) return;
-    if(i == 0) { delta[0] = 0; return; }
+    if(i == 0) return 0;
     const uint64_t x = k[i - 1] ^ k[i];
-    if(x == 0) { delta[i] = (uint8_t)G; return; }
+    if(x == 0) return G;
     const int lz = __clzll((long long)x) - (64 - G * D);
-    delta[i] = (uint8_t)(lz / D);
+    return lz / D;
 }
+
+// Levels of the per-level node lists (> deepest level: max_level + 1 <= 65).
+constexpr int NLEVEL = 72;
+
+// Search groups: compact sets of at most 64 key-consecutive particles.
+// Every cell holding more than 64 particles packs its children that hold at
+// most 64 (cells the tree does not split further, or leaves) into groups,
+// consecutive children filling a group until the next would take it past
+// 64; a child holding more than 64 is packed by its own children, and a
+// leaf holding more than 64 is cut into runs of 64.  A group then lies in
+// one parent cell, so its box is compact, and it is as full as siblings
+// allow (46.7 particles on average at the Evrard benchmark's density
+// profile, against 15.6 for single cells; runs of 64 key-consecutive
+// particles fill the waves but can straddle a jump of the key order across
+// the domain).
+constexpr int GROUP = 64;
 
 struct TreeShape {
     int n, G, D, leaf, max_level;
 };
 
-// First level at which particle b begins a cell.
-__device__ __forceinline__ int first_level(const uint8_t *delta, int b) { return b == 0 ? 2 : (int)delta[b] + 2; }
+// First level at which particle b begins a cell, from its delta.
+__device__ __forceinline__ int first_level(int delta_b, int b) { return b == 0 ? 2 : delta_b + 2; }
 
 // Level-L cells are the runs of sorted keys that share their top L-1 levels
 // (the prefix cell_end compares).  The two helpers below find a run's ends
@@ -209,10 +224,10 @@ __device__ __forceinline__ bool parent_split(const uint64_t *keys, const TreeSha
 }
 
 // Number of nodes that begin at particle b (all at consecutive levels).
-__device__ int nodes_at(const uint64_t *keys, const uint8_t *delta, const TreeShape &ts, int b, int &lvl0)
+__device__ int nodes_at(const uint64_t *keys, int delta_b, const TreeShape &ts, int b, int &lvl0)
 {
-    if(b > 0 && (int)delta[b] >= ts.G) return 0;  // same deepest cell as b-1
-    lvl0 = first_level(delta, b);
+    if(b > 0 && delta_b >= ts.G) return 0;  // same deepest cell as b-1
+    lvl0 = first_level(delta_b, b);
     if(lvl0 > 2 && !parent_split(keys, ts, b, lvl0 - 1)) return 0;
     int k = 0;
     for(int L = lvl0;; ++L) {
@@ -223,12 +238,15 @@ __device__ int nodes_at(const uint64_t *keys, const uint8_t *delta, const TreeSh
     return k;
 }
 
-__global__ void __launch_bounds__(BLOCK) count_nodes_kernel(const uint64_t *keys, const uint8_t *delta, TreeShape ts, int *nn)
+// Nodes beginning at each particle, and the deltas for emit_nodes_kernel.
+__global__ void __launch_bounds__(BLOCK) count_nodes_kernel(const uint64_t *keys, TreeShape ts, uint8_t *delta, int *nn)
 {
     const int b = blockIdx.x * BLOCK + threadIdx.x;
     if(b >= ts.n) return;
+    const int d = key_delta(keys, b, ts.G, ts.D);
+    delta[b] = (uint8_t)d;
     int lvl0 = 0;
-    nn[b] = nodes_at(keys, delta, ts, b, lvl0);
+    nn[b] = nodes_at(keys, d, ts, b, lvl0);
 }
 
 // End (exclusive) of the level-L cell that begins at b: first particle
@@ -247,13 +265,16 @@ __device__ __forceinline__ int cell_end(const uint64_t *keys, const TreeShape &t
     return lo;
 }
 
+// Also clears the per-level counters (lvl) and the search groups: the
+// groups of a root holding at most GROUP particles, none otherwise.
 template <int D>
 __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys, const uint8_t *delta, TreeShape ts, const int *nn,
                                                         const int *noff, const double *scal, TreeDev t, int64_t node_cap,
-                                                        int64_t *counters)
+                                                        int64_t *counters, int *lvl, int2 *groups, int *num_groups)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if(b < 3 * NLEVEL) lvl[b] = 0;  // the grid has at least BLOCK > 3 NLEVEL threads
     if(b >= ts.n) return;
     const int total = 1 + noff[ts.n - 1] + nn[ts.n - 1];
     if(b == 0) {
@@ -261,11 +282,14 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
         if(total > node_cap) atomicAdd((unsigned long long *)&counters[C_TREE_ERR], 1ull);
         t.geo[0] = make_double4(scal[S_ROOT_CX], D > 1 ? scal[S_ROOT_CY] : 0.0, D > 2 ? scal[S_ROOT_CZ] : 0.0, scal[S_ROOT_EDGE]);
         t.rng[0] = make_int4(0, ts.n, total, 1);
+        int ng = 0;
+        if(ts.n <= GROUP) groups[ng++] = make_int2(0, ts.n);
+        *num_groups = ng;
     }
     const int cnt = nn[b];
     if(cnt == 0) return;
     int lvl0 = 0;
-    nodes_at(keys, delta, ts, b, lvl0);
+    nodes_at(keys, (int)delta[b], ts, b, lvl0);
     const int base = 1 + noff[b];
     const uint64_t key = keys[b];
     const uint64_t mask = (1ull << D) - 1ull;
@@ -302,56 +326,13 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
     }
 }
 
-// Parent links and child counts.
-__global__ void __launch_bounds__(BLOCK) link_kernel(TreeDev t, int *parent, int *nchild)
-{
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
-    const int total = *t.num_nodes;
-    if(p >= total) return;
-    if(p == 0) parent[0] = -1;
-    const int4 r = t.rng[p];
-    if(r.w & 256) { nchild[p] = 0; return; }
-    int cnt = 0;
-    for(int c = p + 1; c < r.z && c < total && cnt < 8; c = t.rng[c].z) {
-        parent[c] = p;
-        ++cnt;
-    }
-    nchild[p] = cnt;
-}
-
-__global__ void __launch_bounds__(BLOCK) reset_parent_kernel(int64_t cap, int *parent)
-{
-    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if(p < cap) parent[p] = -1;
-}
-
 // ------------------------------------------------- level-by-level reductions
 // Node sums are formed bottom-up one tree level per launch: a launch
 // boundary orders the children's results before the parents read them, so
 // no cross-XCD atomics or cache maintenance are needed.  The nodes of each
 // level are listed once per tree build (level_bins / level_scatter).
-constexpr int NLEVEL = 72;  // > deepest level (max_level + 1 <= 65)
 
 __device__ __forceinline__ int node_level(const int4 &r) { return (r.w & 255) < NLEVEL ? (r.w & 255) : NLEVEL - 1; }
-
-__global__ void fill_level_kernel(int *lvl)
-{
-    for(int k = threadIdx.x; k < 3 * NLEVEL; k += blockDim.x) lvl[k] = 0;
-}
-
-// per-level node counts (block histograms, one atomic per bin and block)
-__global__ void __launch_bounds__(BLOCK) level_count_kernel(TreeDev t, int *lvl)
-{
-    __shared__ int bins[NLEVEL];
-    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK) bins[k] = 0;
-    __syncthreads();
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
-    const int total = *t.num_nodes;
-    if(p < total) atomicAdd(&bins[node_level(t.rng[p])], 1);
-    __syncthreads();
-    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK)
-        if(bins[k]) atomicAdd(&lvl[k], bins[k]);
-}
 
 // offsets (lvl[NLEVEL + L]) from the counts; clears the fill counters
 __global__ void level_offsets_kernel(int *lvl)
@@ -446,37 +427,26 @@ __global__ void __launch_bounds__(TOP_THREADS) moments_top_kernel(State s, TreeD
     }
 }
 
-// Search groups: compact sets of at most 64 key-consecutive particles.
-// Every cell holding more than 64 particles packs its children that hold at
-// most 64 (cells the tree does not split further, or leaves) into groups,
-// consecutive children filling a group until the next would take it past
-// 64; a child holding more than 64 is packed by its own children, and a
-// leaf holding more than 64 is cut into runs of 64.  A group then lies in
-// one parent cell, so its box is compact, and it is as full as siblings
-// allow (46.7 particles on average at the Evrard benchmark's density
-// profile, against 15.6 for single cells; runs of 64 key-consecutive
-// particles fill the waves but can straddle a jump of the key order across
-// the domain).
-constexpr int GROUP = 64;
 
-__global__ void reset_groups_kernel(TreeDev t, int2 *groups, int *num_groups)
+// Per node: the per-level node counts (block histograms, one atomic per bin
+// and block), and the groups of a packing cell.
+__global__ void __launch_bounds__(BLOCK) level_count_groups_kernel(TreeDev t, int *lvl, int2 *groups, int *num_groups,
+                                                                int64_t group_cap)
 {
-    // a root that is a leaf or holds at most 64 particles is packed here
-    const int4 r = t.rng[0];
-    const int cnt = r.y - r.x;
-    int ng = 0;
-    if(cnt <= GROUP || (r.w & 256)) {
-        for(int b = r.x; b < r.y; b += GROUP) groups[ng++] = make_int2(b, b + GROUP < r.y ? b + GROUP : r.y);
-    }
-    *num_groups = ng;
-}
-
-__global__ void __launch_bounds__(BLOCK) groups_kernel(TreeDev t, int2 *groups, int *num_groups, int64_t group_cap)
-{
+    __shared__ int bins[NLEVEL];
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK) bins[k] = 0;
+    __syncthreads();
     const int q = blockIdx.x * BLOCK + threadIdx.x;
     const int total = *t.num_nodes;
+    int4 rq = make_int4(0, 0, 0, 0);
+    if(q < total) {
+        rq = t.rng[q];
+        atomicAdd(&bins[node_level(rq)], 1);
+    }
+    __syncthreads();
+    for(int k = threadIdx.x; k < NLEVEL; k += BLOCK)
+        if(bins[k]) atomicAdd(&lvl[k], bins[k]);
     if(q >= total) return;
-    const int4 rq = t.rng[q];
     if((rq.w & 256) || rq.y - rq.x <= GROUP) return;  // not a packing cell
     const int end = rq.z > q ? rq.z : total;
     // two passes over the children: count the groups, then write them
@@ -565,8 +535,6 @@ int tree_alloc(Ctx &c)
     SPHGPU_HIP(hipMalloc(&c.t.ks, c.node_cap * sizeof(double)));
     SPHGPU_HIP(hipMalloc(&c.t.rng, c.node_cap * sizeof(int4)));
     SPHGPU_HIP(hipMalloc(&c.t.num_nodes, sizeof(int)));
-    SPHGPU_HIP(hipMalloc(&c.parent, c.node_cap * sizeof(int)));
-    SPHGPU_HIP(hipMalloc(&c.nchild, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.lvl_nodes, c.node_cap * sizeof(int)));
     SPHGPU_HIP(hipMalloc(&c.msum, c.node_cap * sizeof(double4)));
     if(c.p.gravity_is_valid) {
@@ -664,24 +632,19 @@ template <int D> static int tree_build_d(Ctx &c)
     std::swap(c.s.nb, c.alt.nb);
 
     const TreeShape ts{n, G, D, c.p.tree_leaf_particle_num, c.p.tree_max_level};
-    hipLaunchKernelGGL(delta_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, n, G, D, c.delta);
-    hipLaunchKernelGGL(count_nodes_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, c.delta, ts, c.nn);
+    hipLaunchKernelGGL(count_nodes_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, ts, c.delta, c.nn);
     bytes = c.scan_tmp_bytes;
     SPHGPU_HIP(rocprim::exclusive_scan(c.scan_tmp, bytes, c.nn, c.noff, 0, (size_t)n, rocprim::plus<int>(), c.stream));
     hipLaunchKernelGGL(emit_nodes_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, c.keys_sorted, c.delta, ts, c.nn, c.noff,
-                       c.scal, c.t, c.node_cap, c.counters);
+                       c.scal, c.t, c.node_cap, c.counters, c.lvl, c.groups, c.num_groups);
+    // nodes listed by level (in c.lvl_nodes), and the search groups (they
+    // need the node ranges only), so that the level counts and the number of
+    // groups come to the host together
     const int ngrid = grid_for(c.node_cap);
-    hipLaunchKernelGGL(reset_parent_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.node_cap, c.parent);
-    hipLaunchKernelGGL(link_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.parent, c.nchild);
-    // nodes listed by level (in c.lvl_nodes), counts to the host
-    hipLaunchKernelGGL(fill_level_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.lvl);
-    hipLaunchKernelGGL(level_count_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl);
+    hipLaunchKernelGGL(level_count_groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl, c.groups, c.num_groups,
+                       c.group_cap);
     hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.lvl);
     hipLaunchKernelGGL(level_scatter_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl, c.lvl_nodes);
-    // search groups (they need the node ranges only), so that their number
-    // comes to the host with the level counts
-    hipLaunchKernelGGL(reset_groups_kernel, dim3(1), dim3(1), 0, c.stream, c.t, c.groups, c.num_groups);
-    hipLaunchKernelGGL(groups_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.groups, c.num_groups, c.group_cap);
     SPHGPU_HIP(hipMemcpyAsync(c.lvl_host, c.lvl, 2 * NLEVEL * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SPHGPU_HIP(hipMemcpyAsync(c.lvl_host + 2 * NLEVEL, c.num_groups, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
