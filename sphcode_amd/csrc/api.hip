@@ -91,7 +91,7 @@ static void free_buffers(Ctx &c)
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
     fr(c.list); fr(c.list_r); fr(c.list_cnt); fr(c.list_cnt2);
-    fr(c.tile_cand); fr(c.tile_mf); fr(c.tile_mb); fr(c.tile_nc); fr(c.ws_j); fr(c.ws_p);
+    fr(c.tile_cand); fr(c.tile_mf); fr(c.tile_ma); fr(c.tile_nc); fr(c.ws_j); fr(c.ws_p);
     c.tile_wcap = 0;
     c.tile_groups_cap = 0;
     c.ws_waves = 0;

@@ -245,12 +245,12 @@ struct Ctx {
     // t = c % T: tile t's 64 candidate indices sit at slab + 64 t (-1 past
     // the end), and lane l of the group has the bit k of tile_mf[slab + 64 t
     // + l] set when candidate (t, k) is in its front set (the gather
-    // candidates, r < R_l) and of tile_mb[...] when it is one of its other
-    // symmetric candidates.  Consecutive candidates land in different
+    // candidates, r < R_l) and of tile_ma[...] when it is any of its hits
+    // (front or other symmetric candidates).  Consecutive candidates land in different
     // tiles, so each tile samples the whole neighbourhood of the group and
     // the group's lanes hold similar numbers of hits in every tile.
     int *tile_cand = nullptr;
-    uint64_t *tile_mf = nullptr, *tile_mb = nullptr;
+    uint64_t *tile_mf = nullptr, *tile_ma = nullptr;
     int *tile_nc = nullptr;
     int tile_wcap = 0;             // slots per group (multiple of 64)
     int64_t tile_groups_cap = 0;   // groups the slabs are allocated for

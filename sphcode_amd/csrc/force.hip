@@ -19,7 +19,7 @@ struct ForceArgs {
     const int2 *groups;
     int ngroups;
     const int *cand;        // candidate tiles (Ctx::tile_cand)
-    const uint64_t *mf, *mb;
+    const uint64_t *ma;     // hit masks per (tile, lane): every symmetric pair
     const int *gnc;
     int wcap;
     int cap;
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
                     if(a.second_order) l_g[GS ? w : 0][lane] = ((const GradRec<T> *)a.grec)[jl];
             }
             l_j[w][lane] = jl;
-            return valid ? (stream_ld(a.mf + o + lane) | stream_ld(a.mb + o + lane)) : 0ull;
+            return valid ? stream_ld(a.ma + o + lane) : 0ull;
         };
         auto pair = [&](int kc) {
             const int j = l_j[w][kc];
@@ -399,8 +399,7 @@ int force_compute(Ctx &c)
     a.groups = c.groups;
     a.ngroups = c.ngroups;
     a.cand = c.tile_cand;
-    a.mf = c.tile_mf;
-    a.mb = c.tile_mb;
+    a.ma = c.tile_ma;
     a.gnc = c.tile_nc;
     a.wcap = c.tile_wcap;
     a.cap = c.cap;

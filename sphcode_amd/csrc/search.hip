@@ -40,7 +40,7 @@ namespace sphgpu {
 
 struct SearchArgs {
     int *cand;           // group slabs of candidate tiles (Ctx::tile_cand)
-    uint64_t *mf, *mb;   // front / back hit masks per (tile, lane)
+    uint64_t *mf, *ma;   // front / all hit masks per (tile, lane)
     int *gnc;            // candidates per group
     int wcap;            // slots per group slab and per staging buffer
     int *ws_j;           // staging buffers, one per persistent wave
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         }
         const size_t mo = gbase + (size_t)tt * WAVE + lane;
         stream_st(a.mf + mo, mfront);
-        stream_st(a.mb + mo, mine & ~mfront);
+        stream_st(a.ma + mo, mine);
         count += __popcll(mfront);
         count2 += __popcll(mine & ~mfront);
         wave_lds_sync();
@@ -462,7 +462,7 @@ static SearchArgs base_args(Ctx &c)
     SearchArgs a{};
     a.cand = c.tile_cand;
     a.mf = c.tile_mf;
-    a.mb = c.tile_mb;
+    a.ma = c.tile_ma;
     a.gnc = c.tile_nc;
     a.wcap = c.tile_wcap;
     a.ws_j = c.ws_j;
@@ -529,15 +529,15 @@ int search_alloc(Ctx &c)
         const int64_t groups = std::max<int64_t>((int64_t)c.ngroups + c.ngroups / 4 + 64, c.tile_groups_cap);
         if(c.tile_cand) (void)hipFree(c.tile_cand);
         if(c.tile_mf) (void)hipFree(c.tile_mf);
-        if(c.tile_mb) (void)hipFree(c.tile_mb);
+        if(c.tile_ma) (void)hipFree(c.tile_ma);
         if(c.tile_nc) (void)hipFree(c.tile_nc);
         c.tile_cand = nullptr;
-        c.tile_mf = c.tile_mb = nullptr;
+        c.tile_mf = c.tile_ma = nullptr;
         c.tile_nc = nullptr;
         const size_t slots = (size_t)groups * wcap;
         SPHGPU_HIP(hipMalloc(&c.tile_cand, slots * sizeof(int)));
         SPHGPU_HIP(hipMalloc(&c.tile_mf, slots * sizeof(uint64_t)));
-        SPHGPU_HIP(hipMalloc(&c.tile_mb, slots * sizeof(uint64_t)));
+        SPHGPU_HIP(hipMalloc(&c.tile_ma, slots * sizeof(uint64_t)));
         SPHGPU_HIP(hipMalloc(&c.tile_nc, groups * sizeof(int)));
         c.tile_groups_cap = groups;
     }
@@ -552,7 +552,7 @@ int search_alloc(Ctx &c)
 // finds them in.  cnt / cnt2 receive the counts (cnt2 may be null with
 // front_only).  Entries past cap are dropped (the search reports them).
 __global__ void __launch_bounds__(BLOCK) expand_kernel(const int2 *groups, int ngroups, const int *gnc, const int *cand,
-                                                       const uint64_t *mf, const uint64_t *mb, int wcap, int cap,
+                                                       const uint64_t *mf, const uint64_t *ma, int wcap, int cap,
                                                        int front_only, int *list, int *cnt, int *cnt2)
 {
     const int grp = xcd_block() * (BLOCK / WAVE) + (int)(threadIdx.x / WAVE);
@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(BLOCK) expand_kernel(const int2 *groups, int n
             const bool room = wf + wb < cap;
             if(mf[o + lane] & bit) {
                 if(room) list[lb + list_off(wf++)] = cand[o + k];
-            } else if(!front_only && (mb[o + lane] & bit)) {
+            } else if(!front_only && (ma[o + lane] & bit)) {
                 if(room) list[lb + list_off(cap - 1 - wb++)] = cand[o + k];
             }
         }
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(BLOCK) expand_kernel(const int2 *groups, int n
 int search_expand(Ctx &c, int *list, int *cnt, int *cnt2, bool front_only)
 {
     hipLaunchKernelGGL(expand_kernel, dim3(group_grid(c)), dim3(BLOCK), 0, c.stream, c.groups, c.ngroups, c.tile_nc,
-                       c.tile_cand, c.tile_mf, c.tile_mb, c.tile_wcap, c.cap, front_only ? 1 : 0, list, cnt, cnt2);
+                       c.tile_cand, c.tile_mf, c.tile_ma, c.tile_wcap, c.cap, front_only ? 1 : 0, list, cnt, cnt2);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }
