@@ -233,9 +233,11 @@ static int stage_pre(Ctx &c, bool walk = false)
         if((rc = search_gather(c, false))) return rc;
         kmark(c, 1);
     } else {
-        if((rc = search_guess(c))) return rc;
-        if((rc = tree_set_kernel(c))) return rc;  // largest search radius per cell
-        reset_work(c, 0);                         // the search's task counters
+        // the radii h_guess * ratio and the largest radius per cell (set by
+        // sphgpu_integrate's tree build)
+        if(!c.radius_ready && ((rc = search_guess(c)) || (rc = tree_set_kernel(c)))) return rc;
+        c.radius_ready = false;
+        reset_work(c, 0);  // the search's task counters
         // The gravity walk (second stream) runs beside the search.  The
         // search then takes two blocks of four waves per CU, leaving the
         // other wave slots to the walk: the search waits on dependent loads

@@ -245,6 +245,13 @@ template <int D> struct UnitVolume {
     static constexpr double value = D == 1 ? 2.0 : D == 2 ? 3.14159265358979323846 : 4.0 * 3.14159265358979323846 / 3.0;
 };
 
+// The gather search radius sml = h_guess * ratio, h_guess = (N_ngb m /
+// (rho A_D))^(1/D) (src/pre_interaction.cpp:61-64).
+template <int D> __device__ __forceinline__ double sml_guess(double m, double rho, double nngb, double ratio)
+{
+    return root_dim<D>(nngb * m / (rho * UnitVolume<D>::value)) * ratio;
+}
+
 // -------------------------------------------------------- periodic boundary
 // Periodic::calc_r_ij (include/periodic.hpp:34-59): per axis, the smallest
 // of |d|, |d+L|, |d-L| in the reference's tie order.

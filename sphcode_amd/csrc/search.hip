@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
             // p_i.sml = (N_ngb m / (rho A))^(1/D) * ratio (src/pre_interaction.cpp:61-64)
             const double m = s.f[F_MASS][i];
             const double rho = s.f[F_DENS][i];
-            h = root_dim<D>(a.nngb * m / (rho * UnitVolume<D>::value)) * a.ratio;
+            h = sml_guess<D>(m, rho, a.nngb, a.ratio);
             s.f[F_SML][i] = h;
         } else {
             h = s.f[F_SML][i];
@@ -441,7 +441,7 @@ template <int D> __global__ void __launch_bounds__(BLOCK) guess_kernel(State s, 
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
-    s.f[F_SML][i] = root_dim<D>(nngb * s.f[F_MASS][i] / (s.f[F_DENS][i] * UnitVolume<D>::value)) * ratio;
+    s.f[F_SML][i] = sml_guess<D>(s.f[F_MASS][i], s.f[F_DENS][i], nngb, ratio);
 }
 
 int search_guess(Ctx &c)

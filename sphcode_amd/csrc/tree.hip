@@ -130,9 +130,14 @@ struct Reorder {
     double *dst[NF];
     const int *isrc[2];
     int *idst[2];
+    // with sml set: the gather search radius of every particle in the new
+    // order, from its mass and density (the combined search's radius)
+    double *sml;
+    const double *mass, *dens;
+    double nngb, ratio;
 };
 
-__global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *perm, int n, int nfields)
+template <int D> __global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *perm, int n, int nfields)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
@@ -140,6 +145,7 @@ __global__ void __launch_bounds__(BLOCK) reorder_kernel(Reorder r, const int *pe
     for(int f = 0; f < nfields; ++f) r.dst[f][i] = r.src[f][j];
     r.idst[0][i] = r.isrc[0][j];
     if(r.isrc[1]) r.idst[1][i] = r.isrc[1][j];
+    if(r.sml) r.sml[i] = sml_guess<D>(r.mass[j], r.dens[j], r.nngb, r.ratio);
 }
 
 // ----------------------------------------------------------------- delta
@@ -369,10 +375,13 @@ __global__ void __launch_bounds__(BLOCK) level_scatter_kernel(TreeDev t, int *lv
 // particles, an internal node its children (one level deeper, done by the
 // previous launch), in child order.  The root's moments stay zero, as the
 // reference never accumulates them (include/bhtree.hpp:46-53).
-template <int D> __device__ __forceinline__ void moments_node(const State &s, const TreeDev &t, int p, int total, double4 *msum)
+// With KS, also the node's largest smoothing length (kernel_size_node's
+// result) in the same pass.
+template <int D, bool KS> __device__ __forceinline__ void moments_node(const State &s, const TreeDev &t, int p, int total, double4 *msum)
 {
     const int4 r = t.rng[p];
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    double h = 0.0;
     if(r.w & 256) {
         double mx[3] = {0.0, 0.0, 0.0};
         for(int i = r.x; i < r.y; ++i) {
@@ -380,6 +389,10 @@ template <int D> __device__ __forceinline__ void moments_node(const State &s, co
             acc.w += mi;
 #pragma unroll
             for(int q = 0; q < D; ++q) mx[q] += s.f[F_X + q][i] * mi;
+            if constexpr(KS) {
+                const double hi = s.f[F_SML][i];
+                h = hi > h ? hi : h;
+            }
         }
         acc.x = mx[0];
         acc.y = mx[1];
@@ -389,19 +402,24 @@ template <int D> __device__ __forceinline__ void moments_node(const State &s, co
         for(int c = p + 1; c < r.z && c < total && guard < 8; c = t.rng[c].z, ++guard) {
             const double4 v = msum[c];
             acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+            if constexpr(KS) {
+                const double hc = t.ks[c];
+                h = hc > h ? hc : h;
+            }
         }
     }
+    if constexpr(KS) t.ks[p] = h;
     msum[p] = acc;
     if(p == 0) t.mom[0] = make_double4(0.0, 0.0, 0.0, 0.0);
     else t.mom[p] = make_double4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
 }
 
-template <int D>
+template <int D, bool KS>
 __global__ void __launch_bounds__(BLOCK) moments_level_kernel(State s, TreeDev t, const int *nodes, int count, double4 *msum)
 {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if(k >= count) return;
-    moments_node<D>(s, t, nodes[k], *t.num_nodes, msum);
+    moments_node<D, KS>(s, t, nodes[k], *t.num_nodes, msum);
 }
 
 // The upper levels, which hold few nodes, in one launch of one workgroup:
@@ -417,12 +435,12 @@ struct TopLevels {
     int offset[16];
 };
 
-template <int D>
+template <int D, bool KS>
 __global__ void __launch_bounds__(TOP_THREADS) moments_top_kernel(State s, TreeDev t, const int *nodes, TopLevels tl, double4 *msum)
 {
     const int total = *t.num_nodes;
     for(int L = tl.nlev - 1; L >= 0; --L) {
-        for(int k = threadIdx.x; k < tl.count[L]; k += TOP_THREADS) moments_node<D>(s, t, nodes[tl.offset[L] + k], total, msum);
+        for(int k = threadIdx.x; k < tl.count[L]; k += TOP_THREADS) moments_node<D, KS>(s, t, nodes[tl.offset[L] + k], total, msum);
         __syncthreads();
     }
 }
@@ -603,6 +621,7 @@ template <int D> static int tree_build_d(Ctx &c)
 {
     const int n = (int)c.n;
     const int G = c.p.tree_max_level;
+    c.radius_ready = false;
     // root geometry
     if(!c.p.periodic_is_valid) {
         const int nb = std::min(grid_for(n), 1024);
@@ -625,7 +644,17 @@ template <int D> static int tree_build_d(Ctx &c)
     }
     r.isrc[0] = c.s.id; r.idst[0] = c.alt.id;
     r.isrc[1] = c.reorder_light ? nullptr : c.s.nb; r.idst[1] = c.alt.nb;
-    hipLaunchKernelGGL(reorder_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, r, c.perm, n, nf);
+    // sphgpu_integrate's combined search needs the radii h_guess * ratio and
+    // the cells' largest radius: set here, by the reorder and the moments
+    const bool radius = c.reorder_light;
+    if(radius) {
+        r.sml = c.alt.f[F_SML];
+        r.mass = c.s.f[F_MASS];
+        r.dens = c.s.f[F_DENS];
+        r.nngb = (double)c.p.neighbor_number;
+        r.ratio = c.p.iterative_sml ? 1.2 : 1.0;
+    }
+    hipLaunchKernelGGL(reorder_kernel<D>, dim3(grid_for(n)), dim3(BLOCK), 0, c.stream, r, c.perm, n, nf);
     // swap master buffers; FP32 mirrors stay (they are rewritten before use)
     for(int f = 0; f < NF; ++f) std::swap(c.s.f[f], c.alt.f[f]);
     std::swap(c.s.id, c.alt.id);
@@ -654,12 +683,21 @@ template <int D> static int tree_build_d(Ctx &c)
     for(int L = NLEVEL - 1; L >= tl.first + tl.nlev; --L) {
         const int cnt = c.lvl_host[L];
         if(cnt <= 0) continue;
-        hipLaunchKernelGGL(moments_level_kernel<D>, dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
-                           c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
+        if(radius)
+            hipLaunchKernelGGL((moments_level_kernel<D, true>), dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
+                               c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
+        else
+            hipLaunchKernelGGL((moments_level_kernel<D, false>), dim3(grid_for(cnt)), dim3(BLOCK), 0, c.stream, c.s, c.t,
+                               c.lvl_nodes + c.lvl_host[NLEVEL + L], cnt, c.msum);
     }
-    if(tl.nlev > 0)
-        hipLaunchKernelGGL(moments_top_kernel<D>, dim3(1), dim3(TOP_THREADS), 0, c.stream, c.s, c.t, c.lvl_nodes, tl, c.msum);
+    if(tl.nlev > 0) {
+        if(radius)
+            hipLaunchKernelGGL((moments_top_kernel<D, true>), dim3(1), dim3(TOP_THREADS), 0, c.stream, c.s, c.t, c.lvl_nodes, tl, c.msum);
+        else
+            hipLaunchKernelGGL((moments_top_kernel<D, false>), dim3(1), dim3(TOP_THREADS), 0, c.stream, c.s, c.t, c.lvl_nodes, tl, c.msum);
+    }
     SPHGPU_HIP(hipGetLastError());
+    c.radius_ready = radius;
     return search_alloc(c);
 }
 
