@@ -86,7 +86,8 @@ static void free_buffers(Ctx &c)
     fr(c.s.id); fr(c.alt.id); fr(c.s.nb); fr(c.alt.nb);
     fr(c.keys); fr(c.keys_sorted); fr(c.idx); fr(c.perm); fr(c.sort_tmp); fr(c.delta); fr(c.nn); fr(c.noff);
     fr(c.scan_tmp); fr(c.t.geo); fr(c.t.mom); fr(c.t.ks); fr(c.t.rng); fr(c.t.num_nodes);
-    fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.num_groups); fr(c.lvl);
+    fr(c.lvl_nodes); fr(c.msum); fr(c.groups); fr(c.lvl);
+    c.num_groups = nullptr;  // inside lvl
     fr(c.gnode); fr(c.gpart); fr(c.gacc); fr(c.leafhit); fr(c.leafcnt); fr(c.posrec); fr(c.pair_rec); fr(c.grad_rec);
     if(c.lvl_host) (void)hipHostFree(c.lvl_host);
     c.lvl_host = nullptr;
@@ -154,9 +155,16 @@ static int ensure_lists(Ctx &c)
     return SPHGPU_OK;
 }
 
-static int check_flags(Ctx &c, const char *stage)
+// Reads the counters (and, with the scalars, the scalars as well: they
+// share one allocation, so one copy brings both) and reports the errors
+// they record.
+static int check_flags(Ctx &c, const char *stage, bool with_scalars = false)
 {
-    SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    if(with_scalars)
+        SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, SCAL_SPAN + NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                  c.stream));
+    else
+        SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     if(c.counters_host[C_TREE_ERR] > 0)
         return set_error(SPHGPU_E_TREE, std::string(stage) + ": There is no free node.");
@@ -426,14 +434,15 @@ int sphgpu_create(const sphgpu_params *params, void **out)
        (e = hipEventCreateWithFlags(&c->ev_g[1], EV_SCOPE)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_g[2], EV_SCOPE)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_g[3], EV_SCOPE)) != hipSuccess ||
-       (e = hipMalloc(&c->scal, NSCALARS * sizeof(double))) != hipSuccess ||
-       (e = hipMalloc(&c->counters, NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
+       // scalars and counters in one allocation (device and pinned host)
+       (e = hipMalloc(&c->scal, SCAL_SPAN + NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
-       (e = hipHostMalloc(&c->counters_host, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
-       (e = hipHostMalloc(&c->scal_host, NSCALARS * sizeof(double), hipHostMallocDefault)) != hipSuccess) {
+       (e = hipHostMalloc(&c->scal_host, SCAL_SPAN + NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess) {
         delete c;
         return hip_fail(e, "sphgpu_create");
     }
+    c->counters = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal) + SCAL_SPAN);
+    c->counters_host = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal_host) + SCAL_SPAN);
     for(int k = 0; k < 16; ++k) (void)hipEventCreateWithFlags(&c->ev[k], EV_SCOPE);
     hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c->stream, c->counters, NCOUNTERS);
     hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->scal, (int64_t)NSCALARS, 0.0);
@@ -465,12 +474,10 @@ int sphgpu_destroy(void *ptr)
     if(c->gstream) (void)hipStreamSynchronize(c->gstream);
     adjoint_free(*c);
     free_buffers(*c);
-    if(c->scal) (void)hipFree(c->scal);
-    if(c->counters) (void)hipFree(c->counters);
+    if(c->scal) (void)hipFree(c->scal);  // with the counters
     if(c->work) (void)hipFree(c->work);
-    if(c->counters_host) (void)hipHostFree(c->counters_host);
     if(c->counters_spec) (void)hipHostFree(c->counters_spec);
-    if(c->scal_host) (void)hipHostFree(c->scal_host);
+    if(c->scal_host) (void)hipHostFree(c->scal_host);  // with counters_host
     for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     if(c->gstream) (void)hipStreamDestroy(c->gstream);
@@ -722,16 +729,13 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
         return rc;
     }
     mark(5);
-    if(grav) {
-        SPHGPU_HIP(hipStreamWaitEvent(c.stream, c.ev_grav, 0));
-        if((rc = gravity_combine(c, c.stream))) return rc;
-    }
+    if(grav) SPHGPU_HIP(hipStreamWaitEvent(c.stream, c.ev_grav, 0));
     mark(6);
-    if((rc = step_correct(c))) return rc;
+    // the gravity sums are added to acc by the corrector's launch
+    if((rc = step_correct(c, grav))) return rc;
     mark(7);
     c.kernel_marks = false;
-    SPHGPU_HIP(hipMemcpyAsync(c.scal_host, c.scal, NSCALARS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    if((rc = check_flags(c, "integrate"))) return rc;  // synchronises the stream
+    if((rc = check_flags(c, "integrate", true))) return rc;  // synchronises the stream
     c.dt = c.scal_host[S_DT];
     c.h_per_v_sig = c.scal_host[S_HVS];
     c.time += c.dt;
@@ -739,9 +743,10 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     if(c.timers_on) {
         for(int k = 0; k < 7; ++k) {
             float ms = 0.0f;
-            // with gravity on, its timer is the walk's own span on its
-            // stream, which overlaps the force stage; the wait for its end
-            // and the combine (ev[5] to ev[6]) are in no stage
+            // with gravity on, its timer is the walk's and the pair pass's
+            // spans on their stream, which overlap the other stages; the
+            // wait for their end (ev[5] to ev[6]) is in no stage, and their
+            // sums are added to acc in the corrector's launch
             if(k == 5 && grav) {
                 float ms2 = 0.0f;
                 (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);

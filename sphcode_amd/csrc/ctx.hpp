@@ -174,6 +174,9 @@ enum Counter : int {
 enum Scalar : int {
     S_DT = 0, S_HVS, S_ROOT_CX, S_ROOT_CY, S_ROOT_CZ, S_ROOT_EDGE, NSCALARS
 };
+constexpr size_t SCAL_SPAN = 256;  // bytes of the scalars' part of their allocation
+static_assert(NSCALARS * sizeof(double) <= SCAL_SPAN, "scalars fit their span");
+
 
 struct Ctx {
     sphgpu_params p{};
@@ -235,7 +238,7 @@ struct Ctx {
     // search groups: particle ranges of at most 64 particles, each inside
     // the deepest cell holding at most 64 particles
     int2 *groups = nullptr;
-    int *num_groups = nullptr;   // device scalar
+    int *num_groups = nullptr;   // device scalar (an entry of lvl)
     int64_t group_cap = 0;
 
     // candidate tiles of the last neighbour search, per search group g
@@ -279,11 +282,15 @@ struct Ctx {
     bool record = false;
     bool have_gather = false, have_force = false;
 
-    double *scal = nullptr;     // device scalars
+    // device scalars, and the counters SCAL_SPAN bytes further on in the
+    // same allocation (one copy brings both to the host; the counters, which
+    // kernels update with atomics, stay off the cache lines of the scalars,
+    // which every kernel reads), and their pinned host copies laid out alike
+    double *scal = nullptr;
     double *partial = nullptr;  // block partials of reductions
-    int64_t *counters = nullptr;
-    int64_t *counters_host = nullptr;  // pinned
-    double *scal_host = nullptr;       // pinned
+    int64_t *counters = nullptr;       // (char *)scal + SCAL_SPAN
+    int64_t *counters_host = nullptr;  // (char *)scal_host + SCAL_SPAN
+    double *scal_host = nullptr;
 
     // host mirror of simulation scalars
     double time = 0.0, dt = 0.0, h_per_v_sig = 0.0;
@@ -366,7 +373,7 @@ inline int group_grid(const Ctx &c) { return std::max(1, (c.ngroups + BLOCK / WA
 int step_init_fields(Ctx &c);
 int step_timestep(Ctx &c);       // dt into scal[S_DT]
 int step_predict(Ctx &c);
-int step_correct(Ctx &c);
+int step_correct(Ctx &c, bool add_gravity = false);  // add_gravity: acc += the gravity pass's sums first
 int step_energy(Ctx &c, double *out);
 // FP32 mirrors of the listed fields (mixed precision only; no-op in FP64)
 int mirror_fields(Ctx &c, const int *fields, int nfields);

@@ -118,14 +118,24 @@ template <int D> __global__ void __launch_bounds__(BLOCK) predict_kernel(State s
 }
 
 // Solver::correct (src/solver.cpp:458-474).
-template <int D> __global__ void __launch_bounds__(BLOCK) correct_kernel(State s, int n, const double *scal, double cs)
+// With gacc, the gravity acceleration is first added to acc (as
+// gravity_combine_kernel does), in the same launch.
+template <int D> __global__ void __launch_bounds__(BLOCK) correct_kernel(State s, int n, const double *scal, double cs,
+                                                                      const double *gacc, int64_t stride)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if(i >= n) return;
     const double hdt = 0.5 * scal[S_DT];
 #pragma unroll
-    for(int k = 0; k < D; ++k) s.f[F_V + k][i] = s.f[F_VP + k][i] + s.f[F_A + k][i] * hdt;
+    for(int k = 0; k < D; ++k) {
+        double a = s.f[F_A + k][i];
+        if(gacc) {
+            a += gacc[k * stride + i];
+            s.f[F_A + k][i] = a;
+        }
+        s.f[F_V + k][i] = s.f[F_VP + k][i] + a * hdt;
+    }
     const double u = s.f[F_ENEP][i] + s.f[F_DENE][i] * hdt;
     s.f[F_ENE][i] = u;
     s.f[F_SOUND][i] = sqrt(cs * u);
@@ -258,10 +268,11 @@ int step_predict(Ctx &c)
     return SPHGPU_OK;
 }
 
-int step_correct(Ctx &c)
+int step_correct(Ctx &c, bool add_gravity)
 {
     const double cs = c.p.gamma * (c.p.gamma - 1.0);
-    SPHGPU_DISPATCH_D(c, correct_kernel, grid_for(c.n), c.s, (int)c.n, c.scal, cs);
+    SPHGPU_DISPATCH_D(c, correct_kernel, grid_for(c.n), c.s, (int)c.n, c.scal, cs, add_gravity ? c.gacc : nullptr,
+                      (int64_t)c.cap_n);
     SPHGPU_HIP(hipGetLastError());
     return SPHGPU_OK;
 }

@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * BLOCK + threadIdx.x;
-    if(b < 3 * NLEVEL) lvl[b] = 0;  // the grid has at least BLOCK > 3 NLEVEL threads
+    if(b < NLEVEL) lvl[b] = 0;  // the level counts (the grid has at least BLOCK > NLEVEL threads)
     if(b >= ts.n) return;
     const int total = 1 + noff[ts.n - 1] + nn[ts.n - 1];
     if(b == 0) {
@@ -340,6 +340,11 @@ __global__ void __launch_bounds__(BLOCK) emit_nodes_kernel(const uint64_t *keys,
 
 __device__ __forceinline__ int node_level(const int4 &r) { return (r.w & 255) < NLEVEL ? (r.w & 255) : NLEVEL - 1; }
 
+// Layout of lvl: counts [0, NLEVEL), offsets [NLEVEL, 2 NLEVEL), the number
+// of search groups at 2 NLEVEL (so that one copy takes all three to the
+// host), fill counters from LVL_FILL on.
+constexpr int LVL_FILL = 2 * NLEVEL + 1;
+
 // offsets (lvl[NLEVEL + L]) from the counts; clears the fill counters
 __global__ void level_offsets_kernel(int *lvl)
 {
@@ -347,7 +352,7 @@ __global__ void level_offsets_kernel(int *lvl)
     int o = 0;
     for(int k = 0; k < NLEVEL; ++k) {
         lvl[NLEVEL + k] = o;
-        lvl[2 * NLEVEL + k] = 0;
+        lvl[LVL_FILL + k] = 0;
         o += lvl[k];
     }
 }
@@ -366,7 +371,7 @@ __global__ void __launch_bounds__(BLOCK) level_scatter_kernel(TreeDev t, int *lv
     }
     __syncthreads();
     for(int k = threadIdx.x; k < NLEVEL; k += BLOCK)
-        base[k] = bins[k] ? atomicAdd(&lvl[2 * NLEVEL + k], bins[k]) : 0;
+        base[k] = bins[k] ? atomicAdd(&lvl[LVL_FILL + k], bins[k]) : 0;
     __syncthreads();
     if(p < total) nodes[lvl[NLEVEL + L] + base[L] + rank] = p;
 }
@@ -565,13 +570,13 @@ int tree_alloc(Ctx &c)
         SPHGPU_HIP(hipMalloc(&c.leafhit, waves * LEAFHIT_CAP * sizeof(LeafHit)));
         SPHGPU_HIP(hipMalloc(&c.leafcnt, waves * sizeof(int)));
     }
-    if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, 3 * NLEVEL * sizeof(int)));
+    if(!c.lvl) SPHGPU_HIP(hipMalloc(&c.lvl, (LVL_FILL + NLEVEL) * sizeof(int)));
+    c.num_groups = c.lvl + 2 * NLEVEL;
     // per-level counts and offsets, then the number of search groups
     if(!c.lvl_host) SPHGPU_HIP(hipHostMalloc(&c.lvl_host, (2 * NLEVEL + 1) * sizeof(int), hipHostMallocDefault));
     for(int k = 0; k < 2 * NLEVEL + 1; ++k) c.lvl_host[k] = 0;
     c.group_cap = c.node_cap + n / 64 + 1;
     SPHGPU_HIP(hipMalloc(&c.groups, c.group_cap * sizeof(int2)));
-    SPHGPU_HIP(hipMalloc(&c.num_groups, sizeof(int)));
     size_t bytes = 0;
     const unsigned key_bits = (unsigned)(c.p.tree_max_level * c.D);
     SPHGPU_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c.keys, c.keys_sorted, c.idx, c.perm, (unsigned)n, 0, key_bits, c.stream));
@@ -674,8 +679,7 @@ template <int D> static int tree_build_d(Ctx &c)
                        c.group_cap);
     hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.lvl);
     hipLaunchKernelGGL(level_scatter_kernel, dim3(ngrid), dim3(BLOCK), 0, c.stream, c.t, c.lvl, c.lvl_nodes);
-    SPHGPU_HIP(hipMemcpyAsync(c.lvl_host, c.lvl, 2 * NLEVEL * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    SPHGPU_HIP(hipMemcpyAsync(c.lvl_host + 2 * NLEVEL, c.num_groups, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipMemcpyAsync(c.lvl_host, c.lvl, (2 * NLEVEL + 1) * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     c.ngroups = c.lvl_host[2 * NLEVEL];
     if(c.ngroups < 1 || c.ngroups > c.group_cap) return set_error(SPHGPU_E_TREE, "search groups out of range");
