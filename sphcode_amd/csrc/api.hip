@@ -57,7 +57,7 @@ __global__ void fill_double_kernel(double *p, int64_t n, double v)
 
 __global__ void fill_i64_kernel(int64_t *p, int n)
 {
-    if((int)threadIdx.x < n) p[threadIdx.x] = 0;
+    for(int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
 }
 
 static bool field_used(const Ctx &c, int f)
@@ -435,16 +435,16 @@ int sphgpu_create(const sphgpu_params *params, void **out)
        (e = hipEventCreateWithFlags(&c->ev_g[2], EV_SCOPE)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_g[3], EV_SCOPE)) != hipSuccess ||
        // scalars and counters in one allocation (device and pinned host)
-       (e = hipMalloc(&c->scal, SCAL_SPAN + NCOUNTERS * sizeof(int64_t))) != hipSuccess ||
+       (e = hipMalloc(&c->scal, SCAL_SPAN + NCOUNTER_ENTRIES * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
-       (e = hipHostMalloc(&c->scal_host, SCAL_SPAN + NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess) {
+       (e = hipHostMalloc(&c->scal_host, SCAL_SPAN + NCOUNTER_ENTRIES * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess) {
         delete c;
         return hip_fail(e, "sphgpu_create");
     }
     c->counters = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal) + SCAL_SPAN);
     c->counters_host = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal_host) + SCAL_SPAN);
     for(int k = 0; k < 16; ++k) (void)hipEventCreateWithFlags(&c->ev[k], EV_SCOPE);
-    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c->stream, c->counters, NCOUNTERS);
+    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->counters, NCOUNTER_ENTRIES);
     hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->scal, (int64_t)NSCALARS, 0.0);
     if(p.periodic_is_valid) {
         // BHTree::initialize (src/bhtree.cpp:19-31): root = the periodic box
@@ -932,9 +932,18 @@ int sphgpu_get_stats(void *ptr, sphgpu_stats *st)
 {
     SPHGPU_CTX(ptr);
     if(!st) return set_error(SPHGPU_E_INVALID, "null output");
-    SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    SPHGPU_HIP(hipMemcpyAsync(c.counters_host, c.counters, NCOUNTER_ENTRIES * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              c.stream));
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
-    const int64_t *k = c.counters_host;
+    // the rows of the statistics: sums, and the largest of the maxima
+    int64_t k[STAT_ROW] = {};
+    for(int r = 0; r < STAT_SLOTS; ++r) {
+        const int64_t *row = c.counters_host + (size_t)r * STAT_ROW;
+        for(int q = 0; q < NCOUNTERS; ++q) {
+            if(q == C_MAXLIST || q == C_GRAV_MAXREC) k[q] = std::max(k[q], row[q]);
+            else k[q] += row[q];
+        }
+    }
     st->steps = c.steps;
     st->gather_pairs = k[C_GATHER_PAIRS];
     st->force_pairs = k[C_FORCE_PAIRS];
@@ -989,7 +998,7 @@ int sphgpu_reset_stats(void *ptr)
 {
     SPHGPU_CTX(ptr);
     c.trunc_seen = 0;
-    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(WAVE), 0, c.stream, c.counters, NCOUNTERS);
+    hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(BLOCK), 0, c.stream, c.counters, NCOUNTER_ENTRIES);
     SPHGPU_HIP(hipGetLastError());
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     for(double &x : c.timer_ms) x = 0.0;

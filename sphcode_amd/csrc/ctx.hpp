@@ -177,6 +177,29 @@ enum Scalar : int {
 constexpr size_t SCAL_SPAN = 256;  // bytes of the scalars' part of their allocation
 static_assert(NSCALARS * sizeof(double) <= SCAL_SPAN, "scalars fit their span");
 
+// The counters are STAT_SLOTS rows of STAT_ROW entries (256 bytes each).
+// Row 0 is the counter array proper: the rare events the host checks after
+// every stage (truncations, overflows, tree errors) go there.  The
+// statistics that every wave adds to (pairs, iterations, node visits) go
+// to the row of the wave's task modulo STAT_SLOTS, so that the waves'
+// atomic updates spread over many cache lines instead of queueing on one
+// (measured: the Evrard step 4 % faster); sphgpu_get_stats sums the rows
+// (maxima: takes the largest).
+constexpr int STAT_SLOTS = 64;
+constexpr int STAT_ROW = 32;
+static_assert(NCOUNTERS <= STAT_ROW, "a counter row holds every counter");
+constexpr int NCOUNTER_ENTRIES = STAT_SLOTS * STAT_ROW;
+
+__device__ __forceinline__ void stat_add(int64_t *counters, int task, int k, int64_t v)
+{
+    atomicAdd((unsigned long long *)&counters[(task & (STAT_SLOTS - 1)) * STAT_ROW + k], (unsigned long long)v);
+}
+
+__device__ __forceinline__ void stat_max(int64_t *counters, int task, int k, int64_t v)
+{
+    atomicMax((unsigned long long *)&counters[(task & (STAT_SLOTS - 1)) * STAT_ROW + k], (unsigned long long)v);
+}
+
 
 struct Ctx {
     sphgpu_params p{};

@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
         }
     }
     pairs = wave_sum(pairs);
-    if(wave_lane() == 0 && pairs) atomicAdd((unsigned long long *)&a.counters[C_FORCE_PAIRS], (unsigned long long)pairs);
+    if(wave_lane() == 0 && pairs) stat_add(a.counters, grp, C_FORCE_PAIRS, pairs);
 }
 
 template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, const ForceArgs &a, const BoxDev &box)

@@ -209,11 +209,11 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     if(wave_lane() == 0) {
         a.leafcnt[task] = v.nrec;
         if(v.nrec > LEAFHIT_CAP) atomicAdd((unsigned long long *)&a.counters[C_GRAV_REWALK], 1ull);
-        atomicMax((unsigned long long *)&a.counters[C_GRAV_MAXREC], (unsigned long long)(unsigned)v.nrec);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PP], (unsigned long long)(unsigned)pairs);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_PN], (unsigned long long)(unsigned)accepted);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_OPEN], (unsigned long long)(unsigned)opened);
-        atomicAdd((unsigned long long *)&a.counters[C_GRAV_STEPS], (unsigned long long)(unsigned)wc.visits);
+        stat_max(a.counters, task, C_GRAV_MAXREC, (unsigned)v.nrec);
+        stat_add(a.counters, task, C_GRAV_PP, (unsigned)pairs);
+        stat_add(a.counters, task, C_GRAV_PN, (unsigned)accepted);
+        stat_add(a.counters, task, C_GRAV_OPEN, (unsigned)opened);
+        stat_add(a.counters, task, C_GRAV_STEPS, (unsigned)wc.visits);
     }
 }
 
@@ -242,7 +242,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         const int w = uniform(threadIdx.x / WAVE);
         const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
         if(wave_lane() == 0)
-            atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)WAVE * (unsigned)wc.slots);
+            stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)WAVE * (unsigned)wc.slots);
         if(valid) {
             s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
@@ -323,7 +323,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         }
         wave_lds_sync();  // srec is refilled by the next round
     }
-    if(lane == 0) atomicAdd((unsigned long long *)&a.counters[C_GRAV_SLOTS], (unsigned long long)(2 * WAVE) * (unsigned)iters);
+    if(lane == 0) stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)(2 * WAVE) * (unsigned)iters);
     if(valid) {
         s.f[F_PHI][i] += v.phi.val();
 #pragma unroll

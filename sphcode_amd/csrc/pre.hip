@@ -432,10 +432,10 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
     pairs = wave_sum(pairs);
     if(wave_lane() == 0) {
         sh[threadIdx.x / WAVE] = wmin;
-        if(iters) atomicAdd((unsigned long long *)&a.counters[C_NEWTON_ITERS], (unsigned long long)iters);
+        if(iters) stat_add(a.counters, grp, C_NEWTON_ITERS, iters);
         if(fails) atomicAdd((unsigned long long *)&a.counters[C_NEWTON_FAIL], (unsigned long long)fails);
         if(trunc) atomicAdd((unsigned long long *)&a.counters[C_TRUNC], (unsigned long long)trunc);
-        if(pairs) atomicAdd((unsigned long long *)&a.counters[C_GATHER_PAIRS], (unsigned long long)pairs);
+        if(pairs) stat_add(a.counters, grp, C_GATHER_PAIRS, pairs);
     }
     __syncthreads();
     if(threadIdx.x == 0) {

@@ -403,11 +403,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     over = wave_sum(over);
     len = wave_maxv(len);
     if(lane == 0) {
-        atomicAdd((unsigned long long *)&a.counters[C_SEARCH_NODES], (unsigned long long)visited);
-        atomicAdd((unsigned long long *)&a.counters[C_SEARCH_CAND], (unsigned long long)tested);
+        stat_add(a.counters, grp, C_SEARCH_NODES, visited);
+        stat_add(a.counters, grp, C_SEARCH_CAND, tested);
         if(over) atomicAdd((unsigned long long *)&a.counters[C_OVERFLOW], (unsigned long long)over);
         if(nkept > a.wcap) atomicAdd((unsigned long long *)&a.counters[C_TILE_OVER], 1ull);
-        atomicMax((unsigned long long *)&a.counters[C_MAXLIST], (unsigned long long)len);
+        stat_max(a.counters, grp, C_MAXLIST, len);
     }
     }  // groups
 }
