@@ -87,34 +87,42 @@ def cpu_baseline(params, fields, n: int, steps: int, warmup: int = 1) -> dict:
             "host": host_cpu(), "stage_s_per_step": stages}
 
 
-def timed_run(params, fields, steps: int, warmup: int, label: str, gravity_alone: int = 0):
-    """Times `steps` sphgpu_integrate calls after `warmup` untimed ones.
-    With gravity_alone > 0 (and gravity on), the gravity stage is then also
-    timed on its own: that many GravityForce stage calls (walk, pair pass
-    and combine in order on one stream, nothing beside them) on the state
-    the timed steps left, with their interaction counts; the engine is
-    discarded afterwards (each call adds gravity to acc again)."""
-    from sphcode_amd import GPUEngine
-    g = GPUEngine(params)
-    g.upload(fields)
+def _steps(g, fields, steps: int, warmup: int, timers: bool) -> tuple[float, float]:
+    """initialize, `warmup` untimed steps, then `steps` steps: the time of
+    the upload and initialize, and of the steps."""
     t0 = time.perf_counter()
+    g.upload(fields)
     g.initialize()
-    log(f"[bench] {label}: initialize {time.perf_counter() - t0:.2f} s")
-    for k in range(warmup):
+    g.synchronize()
+    t_init = time.perf_counter() - t0
+    for _ in range(warmup):
         g.integrate()
     g.synchronize()
     g.reset_stats()
-    g.enable_timers(True)
+    g.enable_timers(timers)
     g.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         g.integrate()
     g.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats, timers = g.stats(), g.timers()
-    alone = None
-    if gravity_alone > 0 and params.gravity_is_valid:
-        from sphcode_amd import costs
+    return t_init, time.perf_counter() - t0
+
+
+def timed_run(params, fields, steps: int, warmup: int, label: str, gravity_alone: int = 0):
+    """Times `steps` sphgpu_integrate calls after `warmup` untimed ones, with
+    the stage and kernel-span timers (HIP events) on and the diagnostic
+    counters off (their default in the library).  The interaction counts of
+    those steps come from a second run of the same steps with the counters
+    on: every step is deterministic, so its counts are the timed run's.
+    With gravity_alone > 0 (and gravity on), the gravity stage is then also
+    timed on its own: that many GravityForce stage calls (walk, pair pass
+    and combine in order on one stream, nothing beside them) on the state
+    the timed steps left, counted the same way; each engine is discarded
+    afterwards (each call adds gravity to acc again)."""
+    from sphcode_amd import GPUEngine, costs
+    grav = gravity_alone > 0 and params.gravity_is_valid
+
+    def alone_calls(g):
         g.gravity_force()  # untimed: packs the records once
         g.synchronize()
         s0 = g.stats()
@@ -122,17 +130,29 @@ def timed_run(params, fields, steps: int, warmup: int, label: str, gravity_alone
         for _ in range(gravity_alone):
             g.gravity_force()
         g.synchronize()
-        dt_alone = time.perf_counter() - t0
+        dt = time.perf_counter() - t0
         s1 = g.stats()
-        d = {k: s1[k] - s0[k] for k in ("gravity_pp", "gravity_pn", "gravity_opened")}
-        alone = {"calls": gravity_alone, "launch_ms": dt_alone / gravity_alone * 1e3,
-                 "flops_per_launch": costs.gravity_flops(d) / gravity_alone}
+        return dt, {k: s1[k] - s0[k] for k in ("gravity_pp", "gravity_pn", "gravity_opened")}
+
+    g = GPUEngine(params, stats=False)
+    t_init, elapsed = _steps(g, fields, steps, warmup, True)
+    log(f"[bench] {label}: upload and initialize {t_init:.2f} s")
+    timers = g.timers()
+    dt_alone = alone_calls(g)[0] if grav else None
+    g.close()
+    # the same steps with the counters on, for the interaction counts
+    g = GPUEngine(params, stats=True)
+    _steps(g, fields, steps, warmup, False)
+    stats = g.stats()
+    d = alone_calls(g)[1] if grav else None
     g.close()
     n = len(fields["mass"])
     log(f"[bench] {label}: {steps} steps in {elapsed:.3f} s -> {n * steps / elapsed:.4g} particle-steps/s")
     log(f"[bench] {label}: stage ms/step: " + ", ".join(f"{k}={v / steps:.3f}" for k, v in timers.items()))
     log(f"[bench] {label}: stats: {stats}")
-    if alone is not None:
+    if grav:
+        alone = {"calls": gravity_alone, "launch_ms": dt_alone / gravity_alone * 1e3,
+                 "flops_per_launch": costs.gravity_flops(d) / gravity_alone}
         log(f"[bench] {label}: gravity stage alone {alone['launch_ms']:.3f} ms")
         timers = dict(timers)
         timers["_gravity_alone"] = alone
@@ -148,7 +168,7 @@ def adjoint_run(n_side: int, steps: int, dt: float, precision: int) -> dict:
     from sphcode_amd import FP64, GPUEngine, ics
     p, f = ics.rollout(n_side=n_side, precision=precision)
     n = n_side ** 3
-    g = GPUEngine(p)
+    g = GPUEngine(p, stats=False)
 
     def plain():
         g.upload(f)
@@ -206,8 +226,8 @@ def first_step_rate(name: str, params, fields, repeats: int) -> dict:
 
     from sphcode_amd import GPUEngine
     n = int(len(fields["mass"]))
-    g = GPUEngine(params)
-    times, stats, timers = [], None, None
+    g = GPUEngine(params, stats=False)
+    times, timers = [], None
     for r in range(repeats + 1):
         g.upload(fields)
         g.initialize()
@@ -221,7 +241,14 @@ def first_step_rate(name: str, params, fields, repeats: int) -> dict:
         el = time.perf_counter() - t0
         if r > 0:
             times.append(el)
-            stats, timers = g.stats(), g.timers()
+            timers = g.timers()
+    # the same step with the counters on, for its pair count
+    g.enable_stats(True)
+    g.upload(fields)
+    g.initialize()
+    g.reset_stats()
+    g.integrate()
+    stats = g.stats()
     g.close()
     el = statistics.median(times)
     log(f"[bench] {name}: first step {[round(t * 1e3, 2) for t in times]} ms -> {n / el:.4g} particle-steps/s")
@@ -446,6 +473,8 @@ def main() -> int:
         "pairs_per_s": {"gather": stats["gather_pairs"] / args.steps / (ms * 1e-3),
                         "force": stats["force_pairs"] / args.steps / (ms * 1e-3)},
         "stage_ms_per_step": {k: v / args.steps for k, v in timers.items()},
+        "counters": "timed steps run with the diagnostic counters off (the library default); interaction "
+                    "counts come from the same steps rerun with them on",
         "roofline": roofline,
         "kernel_rooflines": kernels,
         "adjoint": adjoint,

@@ -250,6 +250,17 @@ int sphgpu_get_timers(void *ctx, sphgpu_timers *timers);
 int sphgpu_reset_stats(void *ctx);
 int sphgpu_enable_timers(void *ctx, int enable);
 
+/* Diagnostic counters (no counterpart in the reference): with enable != 0
+   the kernels count the interactions, Newton iterations and node visits
+   that sphgpu_get_stats reports (gather_pairs, force_pairs, gravity_pp,
+   gravity_pn, gravity_opened, gravity_wave_steps, gravity_pp_slots,
+   gravity_max_records, newton_iterations, search_nodes,
+   search_candidates, max_list_length).  Off by default: the gravity walk
+   spends scalar instructions on every node visit to count.  The events
+   that change the result or report an error (truncations, list overflows,
+   Newton failures, tree errors, gravity rewalks) are always counted. */
+int sphgpu_enable_stats(void *ctx, int enable);
+
 /* Neighbour sets of the last pre-interaction (kind 0: gather candidates,
    r^2 < (kernel_ratio*h_guess)^2, src/bhtree.cpp:234-270) or the last fluid
    force (kind 1: r < max(h_i,h_j), r != 0, src/fluid_force.cpp:62-64),

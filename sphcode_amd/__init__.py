@@ -35,11 +35,17 @@ def library():
 
 
 class GPUEngine(Engine):
-    """One sphgpu context on the current HIP device."""
+    """One sphgpu context on the current HIP device.  ``stats`` turns on the
+    diagnostic counters that :meth:`stats` reports (interactions, Newton
+    iterations, node visits); the library's default is off."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, stats: bool = True):
         _, fns = library()
         super().__init__(fns, params, name="sphgpu")
+        self.enable_stats(stats)
+
+    def enable_stats(self, on: bool = True) -> None:
+        self._call("enable_stats", C.c_int(1 if on else 0))
 
     def timers(self) -> dict:
         t = _abi.Timers()

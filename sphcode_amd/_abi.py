@@ -195,6 +195,7 @@ GPU_ONLY = {
     "params_default": (C.c_int, [C.POINTER(Params)]),
     "get_timers": (C.c_int, [C.c_void_p, C.POINTER(Timers)]),
     "enable_timers": (C.c_int, [C.c_void_p, C.c_int]),
+    "enable_stats": (C.c_int, [C.c_void_p, C.c_int]),
     "synchronize": (C.c_int, [C.c_void_p]),
     "rollout": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "restart": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),

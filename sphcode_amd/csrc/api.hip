@@ -1013,6 +1013,13 @@ int sphgpu_enable_timers(void *ptr, int enable)
     return SPHGPU_OK;
 }
 
+int sphgpu_enable_stats(void *ptr, int enable)
+{
+    SPHGPU_CTX(ptr);
+    c.stats_on = enable != 0;
+    return SPHGPU_OK;
+}
+
 int sphgpu_neighbor_sets(void *ptr, int kind, int32_t *counts, int32_t *lists, int64_t cap)
 {
     SPHGPU_CTX(ptr);

@@ -216,6 +216,7 @@ struct Ctx {
     bool speculate = false;      // sphgpu_integrate: assume symmetric lists, check after
     bool reorder_light = false;  // sphgpu_integrate's tree build: skip fields the step rewrites
     bool radius_ready = false;   // that build set the search radii and the cells' largest radius
+    bool stats_on = false;       // count the statistics (sphgpu_enable_stats)
     bool flags_pending = false;  // a copy behind ev_flags awaits its check
     int64_t *counters_spec = nullptr;  // pinned host copy for that check
     // called by the force pass between its record packing and its pair

@@ -30,6 +30,7 @@ struct ForceArgs {
     int second_order;
     const double *scal;   // S_DT
     int64_t *counters;
+    int stats;            // add to the statistics counters (errors are always counted)
     int *rec;             // recorded force sets (debug), list_at layout
     int *rec_cnt;
     const double4 *posrec;  // x, y, z, h (FP64)
@@ -361,8 +362,10 @@ __global__ void __launch_bounds__(BLOCK) force_kernel(State s, BoxDev box, int n
             pairs = used;
         }
     }
-    pairs = wave_sum(pairs);
-    if(wave_lane() == 0 && pairs) stat_add(a.counters, grp, C_FORCE_PAIRS, pairs);
+    if(a.stats) {
+        pairs = wave_sum(pairs);
+        if(wave_lane() == 0 && pairs) stat_add(a.counters, grp, C_FORCE_PAIRS, pairs);
+    }
 }
 
 template <int D, int K, int TYPE, class T> static void launch_force(Ctx &c, const ForceArgs &a, const BoxDev &box)
@@ -410,6 +413,7 @@ int force_compute(Ctx &c)
     a.second_order = c.p.gsph_is_2nd_order;
     a.scal = c.scal;
     a.counters = c.counters;
+    a.stats = c.stats_on ? 1 : 0;
     a.rec = c.record ? c.rec_force : nullptr;
     a.rec_cnt = c.record ? c.rec_force_cnt : nullptr;
     a.posrec = c.posrec;

@@ -184,7 +184,7 @@ template <int D, class T, bool RECORD = false> struct GravityVisitor {
 
 // Split pass, part 1: the walk with monopoles only (no h needed), which
 // records the opened leaves per wave for part 2.
-template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_walk_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+template <int D, class T, bool STATS> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_walk_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     const int task = uniform(i / WAVE);
@@ -199,8 +199,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
     __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
     const int w = uniform(threadIdx.x / WAVE);
-    const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
-    const int pairs = wc.pairs, accepted = wc.accepted, opened = wc.opened;
+    const WalkCount wc = tree_walk<STATS>(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
     if(valid) {
         s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
@@ -209,18 +208,20 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
     if(wave_lane() == 0) {
         a.leafcnt[task] = v.nrec;
         if(v.nrec > LEAFHIT_CAP) atomicAdd((unsigned long long *)&a.counters[C_GRAV_REWALK], 1ull);
-        stat_max(a.counters, task, C_GRAV_MAXREC, (unsigned)v.nrec);
-        stat_add(a.counters, task, C_GRAV_PP, (unsigned)pairs);
-        stat_add(a.counters, task, C_GRAV_PN, (unsigned)accepted);
-        stat_add(a.counters, task, C_GRAV_OPEN, (unsigned)opened);
-        stat_add(a.counters, task, C_GRAV_STEPS, (unsigned)wc.visits);
+        if constexpr(STATS) {
+            stat_max(a.counters, task, C_GRAV_MAXREC, (unsigned)v.nrec);
+            stat_add(a.counters, task, C_GRAV_PP, (unsigned)wc.pairs);
+            stat_add(a.counters, task, C_GRAV_PN, (unsigned)wc.accepted);
+            stat_add(a.counters, task, C_GRAV_OPEN, (unsigned)wc.opened);
+            stat_add(a.counters, task, C_GRAV_STEPS, (unsigned)wc.visits);
+        }
     }
 }
 
 // Split pass, part 2: the particle pairs of the recorded leaves (softened
 // with h_i and h_j), added to part 1's sums.  A wave whose record overflowed
 // walks the tree again in full and overwrites part 1's sums.
-template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_leaf_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
+template <int D, class T, bool STATS> __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GravityOcc<T>::value))) gravity_leaf_kernel(State s, TreeDev t, BoxDev box, int n, GravArgs a)
 {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     const int task = uniform(i / WAVE);
@@ -240,9 +241,8 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         __shared__ int stk_node[BLOCK / WAVE][WALK_STACK];
         __shared__ uint64_t stk_mask[BLOCK / WAVE][WALK_STACK];
         const int w = uniform(threadIdx.x / WAVE);
-        const WalkCount wc = tree_walk(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
-        if(wave_lane() == 0)
-            stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)WAVE * (unsigned)wc.slots);
+        const WalkCount wc = tree_walk<STATS>(a.gnode, uniform(*t.num_nodes), valid, v, stk_node[w], stk_mask[w]);
+        if(STATS && wave_lane() == 0) stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)WAVE * (unsigned)wc.slots);
         if(valid) {
             s.f[F_PHI][i] = v.phi.val();
 #pragma unroll
@@ -323,7 +323,7 @@ template <int D, class T> __global__ void __launch_bounds__(BLOCK) __attribute__
         }
         wave_lds_sync();  // srec is refilled by the next round
     }
-    if(lane == 0) stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)(2 * WAVE) * (unsigned)iters);
+    if(STATS && lane == 0) stat_add(a.counters, task, C_GRAV_SLOTS, (int64_t)(2 * WAVE) * (unsigned)iters);
     if(valid) {
         s.f[F_PHI][i] += v.phi.val();
 #pragma unroll
@@ -346,16 +346,33 @@ static GravArgs grav_args(Ctx &c)
     return a;
 }
 
+template <class T, bool STATS> static void walk_d(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
+{
+    const int n = (int)c.n;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL((gravity_walk_kernel<1, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_walk_kernel<2, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_walk_kernel<3, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    }
+}
+
+template <class T, bool STATS> static void leaves_d(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
+{
+    const int n = (int)c.n;
+    switch(c.D) {
+    case 1: hipLaunchKernelGGL((gravity_leaf_kernel<1, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    case 2: hipLaunchKernelGGL((gravity_leaf_kernel<2, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    default: hipLaunchKernelGGL((gravity_leaf_kernel<3, T, STATS>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
+    }
+}
+
 template <class T> static void walk_t(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
 {
     const int n = (int)c.n;
     hipLaunchKernelGGL(pack_nodes_kernel, dim3(std::min<int64_t>(grid_for(c.node_cap), 2048)), dim3(BLOCK), 0, st, c.t,
                        c.gnode, a.G, c.node_cap);
-    switch(c.D) {
-    case 1: hipLaunchKernelGGL((gravity_walk_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    case 2: hipLaunchKernelGGL((gravity_walk_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    default: hipLaunchKernelGGL((gravity_walk_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    }
+    if(c.stats_on) walk_d<T, true>(c, a, box, st);
+    else walk_d<T, false>(c, a, box, st);
 }
 
 template <class T> static void leaves_t(Ctx &c, const GravArgs &a, const BoxDev &box, hipStream_t st)
@@ -371,11 +388,8 @@ template <class T> static void leaves_t(Ctx &c, const GravArgs &a, const BoxDev 
         default: hipLaunchKernelGGL((pack_parts_kernel<3, T>), dim3(grid_for(n + 1)), dim3(BLOCK), 0, st, c.s, n, gp); break;
         }
     }
-    switch(c.D) {
-    case 1: hipLaunchKernelGGL((gravity_leaf_kernel<1, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    case 2: hipLaunchKernelGGL((gravity_leaf_kernel<2, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    default: hipLaunchKernelGGL((gravity_leaf_kernel<3, T>), dim3(grid_for(n)), dim3(BLOCK), 0, st, c.s, c.t, box, n, a); break;
-    }
+    if(c.stats_on) leaves_d<T, true>(c, a, box, st);
+    else leaves_d<T, false>(c, a, box, st);
 }
 
 int gravity_walk(Ctx &c, hipStream_t st)

@@ -45,6 +45,7 @@ struct PreArgs {
     const double *scal;   // S_DT
     int second_order;
     int64_t *counters;
+    int stats;            // add to the statistics counters (errors are always counted)
     double *partial;      // per-block min of h / v_sig
     const void *prec;     // PreRec<T> per particle
     const double4 *pm;    // FP64 position and mass per particle (Newton sweeps)
@@ -432,10 +433,10 @@ __global__ void __launch_bounds__(BLOCK) pre_kernel(State s, BoxDev box, int n, 
     pairs = wave_sum(pairs);
     if(wave_lane() == 0) {
         sh[threadIdx.x / WAVE] = wmin;
-        if(iters) stat_add(a.counters, grp, C_NEWTON_ITERS, iters);
+        if(a.stats && iters) stat_add(a.counters, grp, C_NEWTON_ITERS, iters);
         if(fails) atomicAdd((unsigned long long *)&a.counters[C_NEWTON_FAIL], (unsigned long long)fails);
         if(trunc) atomicAdd((unsigned long long *)&a.counters[C_TRUNC], (unsigned long long)trunc);
-        if(pairs) stat_add(a.counters, grp, C_GATHER_PAIRS, pairs);
+        if(a.stats && pairs) stat_add(a.counters, grp, C_GATHER_PAIRS, pairs);
     }
     __syncthreads();
     if(threadIdx.x == 0) {
@@ -466,6 +467,7 @@ static PreArgs make_args(Ctx &c)
     a.scal = c.scal;
     a.second_order = c.p.gsph_is_2nd_order;
     a.counters = c.counters;
+    a.stats = c.stats_on ? 1 : 0;
     a.partial = c.partial;
     a.prec = c.pair_rec;
     a.pm = c.posrec;  // free during the pre-interaction (the force pass repacks it)

@@ -52,6 +52,7 @@ struct SearchArgs {
     double ratio;
     double nngb;
     int64_t *counters;
+    int stats;            // add to the statistics counters (errors are always counted)
     int *work;           // per-XCD task counters, zeroed before the launch
 };
 
@@ -403,11 +404,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
     over = wave_sum(over);
     len = wave_maxv(len);
     if(lane == 0) {
-        stat_add(a.counters, grp, C_SEARCH_NODES, visited);
-        stat_add(a.counters, grp, C_SEARCH_CAND, tested);
+        if(a.stats) {
+            stat_add(a.counters, grp, C_SEARCH_NODES, visited);
+            stat_add(a.counters, grp, C_SEARCH_CAND, tested);
+            }
         if(over) atomicAdd((unsigned long long *)&a.counters[C_OVERFLOW], (unsigned long long)over);
         if(nkept > a.wcap) atomicAdd((unsigned long long *)&a.counters[C_TILE_OVER], 1ull);
-        stat_max(a.counters, grp, C_MAXLIST, len);
     }
     }  // groups
 }
@@ -469,6 +471,7 @@ static SearchArgs base_args(Ctx &c)
     a.ws_p = c.ws_p;
     a.cap = c.cap;
     a.counters = c.counters;
+    a.stats = c.stats_on ? 1 : 0;
     return a;
 }
 

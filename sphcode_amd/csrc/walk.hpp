@@ -38,17 +38,19 @@ constexpr int WALK_STACK = 72;   // > deepest possible tree (64 levels)
 // The walk state (cursor, lane masks, parked stack) is wave-uniform and kept
 // in scalar registers; lanes take their part of a mask with lane_in, so no
 // per-lane flag is carried from one visit to the next.  Each node is one
-// 64-byte record (GravNode) read with one scalar load.  WalkCount returns
-// the wave's node visits, the particles of the leaves that some lane
-// opened, and the lanes' opened nodes, accepted nodes and particle
-// interactions, all wave totals counted from the lane masks with scalar
-// bit counts (no vector instructions per visit).
+// 64-byte record (GravNode) read with one scalar load.  With STATS,
+// WalkCount returns the wave's node visits, the particles of the leaves
+// that some lane opened, and the lanes' opened nodes, accepted nodes and
+// particle interactions, all wave totals counted from the lane masks with
+// scalar bit counts (no vector instructions per visit, but five scalar
+// ones: the gravity stage alone took 2.85 against 2.66 ms without them at
+// the Evrard benchmark, so they are counted only on request).
 struct WalkCount {
     int visits = 0, slots = 0;
     int opened = 0, accepted = 0, pairs = 0;
 };
 
-template <class V>
+template <bool STATS, class V>
 __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bool valid, V &v, int *stk_node,
                                                uint64_t *stk_mask)
 {
@@ -80,7 +82,7 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
     if(active == 0 || total <= 0) return wc;
     const char *gnb = reinterpret_cast<const char *>(gn);
     do {
-        ++wc.visits;
+        if constexpr(STATS) ++wc.visits;
         // the whole 64-byte record in one scalar load (the compiler would
         // otherwise split it by the fields used into five); the empty asm
         // statement only keeps all sixteen words live
@@ -93,15 +95,19 @@ __device__ __forceinline__ WalkCount tree_walk(const GravNode *gn, int total, bo
         }
         const int skip = g.skip;
         const uint64_t om = v.node(g, active);
-        const int nopen = __popcll(om);
-        wc.opened += nopen;
-        wc.accepted += __popcll(active) - nopen;
+        if constexpr(STATS) {
+            const int nopen = __popcll(om);
+            wc.opened += nopen;
+            wc.accepted += __popcll(active) - nopen;
+        }
         if(om == 0) {
             node = skip;
         } else if(g.leaf) {
             v.leaf(g.begin, g.end, om);
-            wc.pairs += nopen * (g.end - g.begin);
-            wc.slots += g.end - g.begin;
+            if constexpr(STATS) {
+                wc.pairs += __popcll(om) * (g.end - g.begin);
+                wc.slots += g.end - g.begin;
+            }
             node = skip;
         } else {
             const uint64_t fm = active & ~om;
