@@ -407,7 +407,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_
         if(a.stats) {
             stat_add(a.counters, grp, C_SEARCH_NODES, visited);
             stat_add(a.counters, grp, C_SEARCH_CAND, tested);
-            }
+            stat_max(a.counters, grp, C_MAXLIST, len);
+        }
         if(over) atomicAdd((unsigned long long *)&a.counters[C_OVERFLOW], (unsigned long long)over);
         if(nkept > a.wcap) atomicAdd((unsigned long long *)&a.counters[C_TILE_OVER], 1ull);
     }

@@ -231,6 +231,36 @@ def test_gravity_interaction_counts(gpu, case):
     gpu_e.close()
 
 
+def test_stats_switch(gpu):
+    """The diagnostic counters change no result: two steps with them off
+    (the library default) reproduce the counted run bit for bit and report
+    no interaction counts, while the counted run reports all of them.  Its
+    longest candidate list bounds every particle's neighbour count (the
+    gather set within the final h lies in the candidates within
+    1.2 h_guess when no search was truncated)."""
+    from sphcode_amd import GPUEngine, ics
+    p, f = ics.evrard(4096, leaf=32)
+    on, off = GPUEngine(p, stats=True), GPUEngine(p, stats=False)
+    for e in (on, off):
+        e.upload(f)
+        e.initialize()
+        e.integrate()
+        e.integrate()
+    a, b = on.download(), off.download()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    sa, sb = on.stats(), off.stats()
+    print(f"[stats] counted: {sa}", flush=True)
+    for k in ("gather_pairs", "force_pairs", "gravity_pp", "gravity_pn", "gravity_opened", "newton_iterations",
+              "search_nodes", "search_candidates", "gravity_wave_steps", "gravity_pp_slots", "gravity_max_records",
+              "max_list_length"):
+        assert sa[k] > 0 and sb[k] == 0, f"{k}: counted {sa[k]}, uncounted {sb[k]}"
+    assert sa["search_truncations"] == 0
+    assert sa["max_list_length"] >= int(np.max(a["neighbor"]))
+    on.close()
+    off.close()
+
+
 def test_search_truncation_fallback(gpu):
     """A homologous expansion v = kappa x over one fixed step with
     kappa dt = 0.26: the density falls to half, so the smoothing length of
