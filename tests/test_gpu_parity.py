@@ -209,9 +209,11 @@ def test_gravity_interaction_counts(gpu, case):
     """The wave-coherent walk keeps each particle's own opening decisions, so
     it opens, accepts and pairs what the reference's recursive walk does
     (src/bhtree.cpp:301-331): the counts of opened cells, accepted cells and
-    particle pairs match the oracle's (up to the rare cell whose opening test
-    is decided by the last bit of a centre of mass summed in another order).
-    The pair pass's lane slots cover every pair, two per lane and pass."""
+    particle pairs equal the oracle's.  Both sum each centre of mass over the
+    children in slot order and test edge^2 > theta^2 d^2 with d^2 summed left
+    to right without contraction, so not even a cell at the boundary of the
+    opening test may differ.  The pair pass's lane slots cover every pair, two
+    per lane and pass."""
     p, f = CASES[case]()
     ref = OracleEngine(p)
     from sphcode_amd import GPUEngine
@@ -225,7 +227,7 @@ def test_gravity_interaction_counts(gpu, case):
           flush=True)
     for k in ("gravity_pp", "gravity_pn", "gravity_opened"):
         assert a[k] > 0
-        assert abs(b[k] - a[k]) <= 1e-4 * a[k], f"{case}: {k} {b[k]} against {a[k]}"
+        assert b[k] == a[k], f"{case}: {k} {b[k]} against {a[k]}"
     assert b["gravity_rewalks"] == 0
     assert b["gravity_pp_slots"] >= b["gravity_pp"] and b["gravity_pp_slots"] % 128 == 0
     gpu_e.close()
