@@ -430,10 +430,6 @@ int sphgpu_create(const sphgpu_params *params, void **out)
        (e = hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming | EV_SCOPE)) != hipSuccess ||
        (e = hipEventCreateWithFlags(&c->ev_flags, hipEventDisableTiming)) != hipSuccess ||
        (e = hipHostMalloc(&c->counters_spec, NCOUNTERS * sizeof(int64_t), hipHostMallocDefault)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_g[0], EV_SCOPE)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_g[1], EV_SCOPE)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_g[2], EV_SCOPE)) != hipSuccess ||
-       (e = hipEventCreateWithFlags(&c->ev_g[3], EV_SCOPE)) != hipSuccess ||
        // scalars and counters in one allocation (device and pinned host)
        (e = hipMalloc(&c->scal, SCAL_SPAN + NCOUNTER_ENTRIES * sizeof(int64_t))) != hipSuccess ||
        (e = hipMalloc(&c->work, WORK_SLOTS * NUM_XCD * sizeof(int))) != hipSuccess ||
@@ -443,7 +439,7 @@ int sphgpu_create(const sphgpu_params *params, void **out)
     }
     c->counters = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal) + SCAL_SPAN);
     c->counters_host = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(c->scal_host) + SCAL_SPAN);
-    for(int k = 0; k < 16; ++k) (void)hipEventCreateWithFlags(&c->ev[k], EV_SCOPE);
+
     hipLaunchKernelGGL(fill_i64_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->counters, NCOUNTER_ENTRIES);
     hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(BLOCK), 0, c->stream, c->scal, (int64_t)NSCALARS, 0.0);
     if(p.periodic_is_valid) {
@@ -478,10 +474,12 @@ int sphgpu_destroy(void *ptr)
     if(c->work) (void)hipFree(c->work);
     if(c->counters_spec) (void)hipHostFree(c->counters_spec);
     if(c->scal_host) (void)hipHostFree(c->scal_host);  // with counters_host
-    for(int k = 0; k < 16; ++k) (void)hipEventDestroy(c->ev[k]);
+    for(int t = 0; t < TIMER_SETS; ++t)
+        for(int k = 0; k < TIMER_EVENTS; ++k)
+            if(c->tev[t][k]) (void)hipEventDestroy(c->tev[t][k]);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     if(c->gstream) (void)hipStreamDestroy(c->gstream);
-    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_flags, c->ev_g[0], c->ev_g[1], c->ev_g[2], c->ev_g[3]})
+    for(hipEvent_t e : {c->ev_pre, c->ev_grav, c->ev_flags})
         if(e) (void)hipEventDestroy(e);
     delete c;
     return SPHGPU_OK;
@@ -678,11 +676,58 @@ int sphgpu_initialize(void *ptr)
     return sphgpu_gravity_force(ptr);
 }
 
+// Adds the spans of the pending timer sets (steps that have completed) to
+// timer_ms, oldest first.
+static void timers_flush(Ctx &c)
+{
+    for(; c.tset_pending > 0; --c.tset_pending) {
+        const int t = (c.tset_cur - c.tset_pending + TIMER_SETS) % TIMER_SETS;
+        hipEvent_t *ev = c.tev[t], *eg = c.tev[t] + 16;
+        const bool grav = c.tset_grav[t];
+        for(int k = 0; k < 7; ++k) {
+            float ms = 0.0f;
+            // with gravity on, its timer is the walk's and the pair pass's
+            // spans on their stream, which overlap the other stages; the
+            // wait for their end (ev[5] to ev[6]) is in no stage, and their
+            // sums are added to acc in the corrector's launch
+            if(k == 5 && grav) {
+                float ms2 = 0.0f;
+                (void)hipEventElapsedTime(&ms, eg[0], eg[1]);
+                (void)hipEventElapsedTime(&ms2, eg[2], eg[3]);
+                ms += ms2;
+            } else {
+                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            }
+            c.timer_ms[k] += ms;
+        }
+        // kernel spans: search, pre-interaction pairs, force pairs
+        for(int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[8 + 2 * k], ev[9 + 2 * k]);
+            c.timer_ms[7 + k] += ms;
+        }
+        if(grav) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, eg[0], eg[1]);
+            c.timer_ms[10] += ms;
+            (void)hipEventElapsedTime(&ms, eg[2], eg[3]);
+            c.timer_ms[11] += ms;
+        }
+    }
+}
+
 int sphgpu_integrate(void *ptr, double fixed_dt)
 {
     SPHGPU_CTX(ptr);
     SPHGPU_NEED_TREE();
     int rc;
+    if(c.timers_on) {
+        // this step's timer set (the oldest pending ones are read first
+        // when every set is in use)
+        if(c.tset_pending == TIMER_SETS) timers_flush(c);
+        c.ev = c.tev[c.tset_cur];
+        c.ev_g = c.tev[c.tset_cur] + 16;
+    }
     hipEvent_t *ev = c.ev;
     auto mark = [&](int k) {
         if(c.timers_on) (void)hipEventRecord(ev[k], c.stream);
@@ -741,35 +786,10 @@ int sphgpu_integrate(void *ptr, double fixed_dt)
     c.time += c.dt;
     ++c.steps;
     if(c.timers_on) {
-        for(int k = 0; k < 7; ++k) {
-            float ms = 0.0f;
-            // with gravity on, its timer is the walk's and the pair pass's
-            // spans on their stream, which overlap the other stages; the
-            // wait for their end (ev[5] to ev[6]) is in no stage, and their
-            // sums are added to acc in the corrector's launch
-            if(k == 5 && grav) {
-                float ms2 = 0.0f;
-                (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
-                (void)hipEventElapsedTime(&ms2, c.ev_g[2], c.ev_g[3]);
-                ms += ms2;
-            } else {
-                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-            }
-            c.timer_ms[k] += ms;
-        }
-        // kernel spans: search, pre-interaction pairs, force pairs
-        for(int k = 0; k < 3; ++k) {
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ev[8 + 2 * k], ev[9 + 2 * k]);
-            c.timer_ms[7 + k] += ms;
-        }
-        if(grav) {
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, c.ev_g[0], c.ev_g[1]);
-            c.timer_ms[10] += ms;
-            (void)hipEventElapsedTime(&ms, c.ev_g[2], c.ev_g[3]);
-            c.timer_ms[11] += ms;
-        }
+        // read when the timers are (sphgpu_get_timers)
+        c.tset_grav[c.tset_cur] = grav;
+        c.tset_cur = (c.tset_cur + 1) % TIMER_SETS;
+        ++c.tset_pending;
     }
     return SPHGPU_OK;
 }
@@ -979,6 +999,9 @@ int sphgpu_get_timers(void *ptr, sphgpu_timers *t)
 {
     SPHGPU_CTX(ptr);
     if(!t) return set_error(SPHGPU_E_INVALID, "null output");
+    SPHGPU_HIP(hipStreamSynchronize(c.stream));
+    if(c.gstream) SPHGPU_HIP(hipStreamSynchronize(c.gstream));
+    timers_flush(c);
     t->timestep = c.timer_ms[0];
     t->predict = c.timer_ms[1];
     t->tree = c.timer_ms[2];
@@ -1002,6 +1025,7 @@ int sphgpu_reset_stats(void *ptr)
     SPHGPU_HIP(hipGetLastError());
     SPHGPU_HIP(hipStreamSynchronize(c.stream));
     for(double &x : c.timer_ms) x = 0.0;
+    c.tset_pending = 0;  // steps before the reset are not counted
     c.steps = 0;
     return SPHGPU_OK;
 }
@@ -1009,6 +1033,12 @@ int sphgpu_reset_stats(void *ptr)
 int sphgpu_enable_timers(void *ptr, int enable)
 {
     SPHGPU_CTX(ptr);
+    if(enable && !c.tev[0][0]) {
+        for(int t = 0; t < TIMER_SETS; ++t)
+            for(int k = 0; k < TIMER_EVENTS; ++k) SPHGPU_HIP(hipEventCreateWithFlags(&c.tev[t][k], EV_SCOPE));
+        c.ev = c.tev[0];
+        c.ev_g = c.tev[0] + 16;
+    }
     c.timers_on = enable != 0;
     return SPHGPU_OK;
 }

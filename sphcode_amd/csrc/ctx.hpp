@@ -201,6 +201,9 @@ __device__ __forceinline__ void stat_max(int64_t *counters, int task, int k, int
 }
 
 
+constexpr int TIMER_SETS = 32;
+constexpr int TIMER_EVENTS = 20;
+
 struct Ctx {
     sphgpu_params p{};
     int D = 3;
@@ -208,7 +211,8 @@ struct Ctx {
     // second stream (default priority) on which sphgpu_integrate runs the
     // gravity pass alongside the force pass
     hipStream_t gstream = nullptr;
-    hipEvent_t ev_pre = nullptr, ev_grav = nullptr, ev_g[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_pre = nullptr, ev_grav = nullptr;
+    hipEvent_t *ev_g = nullptr;  // the gravity stream's timing events of the current timer set
     // sphgpu_integrate's check of the combined lists, read without stalling
     // the stream: the counters are copied after the pre-interaction (event
     // ev_flags) and read once the force pass is queued
@@ -338,7 +342,15 @@ struct Ctx {
     // stage timers
     bool timers_on = false;
     bool kernel_marks = false;  // sphgpu_integrate with timers on: mark kernel spans
-    hipEvent_t ev[16];          // ev[0..7] stage boundaries, ev[8..13] kernel spans
+    // sphgpu_integrate's timers: each timed step records into one of
+    // TIMER_SETS sets of events (ev[0..7] stage boundaries, ev[8..13]
+    // kernel spans, then the gravity stream's four), and the spans are
+    // summed into timer_ms only when the timers are read or the sets run
+    // out, so that no step waits on the host reading the previous one's
+    hipEvent_t tev[TIMER_SETS][TIMER_EVENTS] = {};  // created when timers are first enabled
+    bool tset_grav[TIMER_SETS] = {};
+    int tset_cur = 0, tset_pending = 0;  // pending sets end at tset_cur
+    hipEvent_t *ev = nullptr;            // the current set
     double timer_ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t steps = 0;
     int64_t last_nodes = 0, last_levels = 0;
