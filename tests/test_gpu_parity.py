@@ -389,3 +389,37 @@ def test_sedov_config_contrast_first_step(gpu):
         assert _rel(b[k][ok], a[k][ok]) <= 1e-8, k
     assert _rms_err(b["vel"][ok], a["vel"][ok]) <= 1e-8
     gpu_e.close()
+
+
+def test_stage_timers_over_many_steps(gpu):
+    """The stage timers of sphgpu_integrate record each step into one of 32
+    event sets and are read when asked for: over 40 steps (the sets wrap)
+    every stage and kernel span is counted once per step, reading them
+    twice adds nothing, the spans are consistent (the kernel spans lie
+    inside their stages), and a reset clears them."""
+    from sphcode_amd import GPUEngine, ics
+    p, f = ics.evrard(4096, leaf=32)
+    g = GPUEngine(p, stats=False)
+    g.upload(f)
+    g.initialize()
+    g.enable_timers(True)
+    g.reset_stats()
+    for _ in range(5):
+        g.integrate()
+    t5 = g.timers()
+    assert g.timers() == t5  # reading again adds nothing
+    for _ in range(35):
+        g.integrate()
+    t40 = g.timers()
+    print(f"[timers] 5 steps {t5}\n[timers] 40 steps {t40}", flush=True)
+    for k in ("timestep", "predict", "tree", "pre_interaction", "fluid_force", "gravity", "correct", "search",
+              "pre_pairs", "force_pairs", "gravity_walk", "gravity_leaf"):
+        assert t5[k] > 0.0, k
+        # 35 more steps of the same kind: several times the first five
+        assert 3.0 * t5[k] < t40[k] < 20.0 * t5[k], f"{k}: {t5[k]} after 5 steps, {t40[k]} after 40"
+    assert t40["search"] + t40["pre_pairs"] <= t40["pre_interaction"] * 1.001
+    assert t40["force_pairs"] <= t40["fluid_force"] * 1.001
+    assert abs(t40["gravity"] - t40["gravity_walk"] - t40["gravity_leaf"]) <= 1e-6 * t40["gravity"]
+    g.reset_stats()
+    assert all(v == 0.0 for v in g.timers().values())
+    g.close()
