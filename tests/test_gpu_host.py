@@ -135,6 +135,44 @@ def test_restart_continues_run(gpu, tmp_path, case):
         assert err < 1e-10, f"{case}: {k} max rel err {err:.3e}"
 
 
+@pytest.mark.parametrize("case", ["sod_gsph2", "khi_disph", "evrard_disph"])
+def test_cli_restart_matches_oracle(gpu, tmp_path, case):
+    """The driver's path end to end against the oracle: sphgpu_run for 3
+    steps with --output, then 3 more with --restart from that snapshot
+    (src/solver.cpp:318-345 loop, src/output.cpp snapshot), against the
+    oracle's initialize + 6 uninterrupted steps, with the bars the fused
+    path's own 3-step oracle comparison uses."""
+    from _oracle import OracleEngine
+    p, f = CASES[case]()
+    p.time_end = 1.0
+    pj, sin = tmp_path / "p.json", tmp_path / "in.snap"
+    snapshot.write_params(pj, p)
+    snapshot.write(sin, f, int(p.dim))
+    _cli(tmp_path, pj, sin, tmp_path / "three.snap", 3, False)
+    got, _, t, dt, _ = _cli(tmp_path, pj, tmp_path / "three.snap", tmp_path / "cont.snap", 3, True)
+    ref = OracleEngine(p)
+    ref.upload(f)
+    ref.initialize()
+    for _ in range(6):
+        ref.integrate()
+    t_ref, dt_ref = ref.time()[:2]
+    want = ref.download()
+    ref.close()
+    assert abs(t - t_ref) <= 1e-9 * t_ref and abs(dt - dt_ref) <= 1e-9 * dt_ref, f"{case}: t/dt {t} {dt} vs {t_ref} {dt_ref}"
+    # both keep the caller's particle order; align by id regardless
+    g, w = np.argsort(got["id"], kind="stable"), np.argsort(want["id"], kind="stable")
+    np.testing.assert_array_equal(got["id"][g], want["id"][w])
+    n, d = len(want["id"]), int(p.dim)
+    for k in ("sml", "dens", "pres"):
+        a, b = np.asarray(got[k]).reshape(n)[g], want[k].reshape(n)[w]
+        err = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+        assert err <= 1e-6, f"{case}: {k} rel err {err:.3e}"
+    for k, bar in (("pos", 1e-6), ("vel", 1e-6), ("acc", 1e-5)):
+        a, b = np.asarray(got[k]).reshape(n, d)[g], want[k].reshape(n, d)[w]
+        err = np.max(np.abs(a - b)) / (np.sqrt(np.mean(b * b)) + 1e-300)
+        assert err <= bar, f"{case}: {k} err {err:.3e}"
+
+
 def test_restart_rejects_fresh_snapshot(gpu, tmp_path):
     """--restart needs the min h/v_sig of a finished step; an initial
     condition written without it is refused."""
